@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Throughput of the phase-linking path on one MI355X.
+
+Workload (one step = one pass of the path over the whole frame, inputs
+resident in HBM): N=30 dates, 2048 x 2048 pixels, Voronoi decorrelation
+scene (16 cells, gamma0 ~ U(0.05, 0.95), tau ~ U(24, 120) d, seed 0),
+21x7 window, KS test (alpha 0.05), EVD, min_shp 30, bias correction,
+consecutive pair maps. The timed path is tcoh_run_device (prep, SHP,
+covariance, eigen) in the mixed precision form; the plain FP32 form runs
+the same workload for the ratio.
+
+Reported beside the GPU figure:
+  roofline      the dominant kernel's algorithmic FLOP rate (from HIP
+                events on the handle's stream over the timed steps)
+                against the MI355X FP32 peak
+  cpu_baseline  the CPU oracle (plain C restatement of the reference
+                kernel, OpenMP over rows) on a band of the same frame
+  accuracy      max/RMS |d gamma| and |d pair| of both forms against the
+                FP64 oracle on a crop, and SHP-count mismatches there
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpixel/s at N=30, 21x7 KS+EVD: mixed-prec vs plain FP32 GPU, |dγ|<=1e-4"
+PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
+PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def prepare_on_device(torch, stack, ha, hr):
+    """read_strip (main.c:52-127) on the GPU for one whole-frame strip."""
+    n, rows, cols = stack.shape
+    pr, pc = rows + 2 * ha, cols + 2 * hr
+    x = torch.zeros((pr, pc, n), dtype=torch.complex64, device=stack.device)
+    x[ha:ha + rows, hr:hr + cols] = stack.permute(1, 2, 0)
+    re, im = x.real.contiguous(), x.imag.contiguous()
+    inside = torch.zeros((pr, pc), dtype=torch.bool, device=stack.device)
+    inside[ha:ha + rows, hr:hr + cols] = True
+    valid = inside & ((re != 0) | (im != 0)).all(dim=-1)
+    amp = torch.sqrt(re * re + im * im)
+    amp = torch.sort(amp, dim=-1).values
+    vf = valid.unsqueeze(-1)
+    amp = torch.where(vf, amp, torch.zeros_like(amp)).contiguous()
+    slc = torch.view_as_real(torch.where(vf, x, torch.zeros_like(x))).contiguous()
+    return slc, amp, valid.to(torch.uint8).contiguous()
+
+
+def flops_eigen(n):
+    """Algorithmic FLOPs of one estimated pixel's eigen stage: Hermitian
+    tridiagonal reduction (16/3 N^3) plus normalisation and fit 8N(N-1)."""
+    return 16.0 / 3.0 * n ** 3 + 8.0 * n * (n - 1)
+
+
+def run_form(tcoh, s, pcols, bufs, rows, cols, precision, steps, warmup, torch):
+    slc, amp, valid, coh, count, pairs = bufs
+    be = tcoh.Backend(s, pcols, platform=0, device=0, precision=precision)
+    pr = slc.shape[0]
+    args = (slc.data_ptr(), amp.data_ptr(), valid.data_ptr(), pr, rows, cols,
+            coh.data_ptr(), count.data_ptr(), pairs.data_ptr())
+    for _ in range(warmup):
+        be.run_device(*args)
+    torch.cuda.synchronize()
+    be.set_timing(True)
+    stage_ms = np.zeros(4)
+    launches = np.zeros(4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        be.run_device(*args)
+        ms, nl = be.stage_times()
+        stage_ms += ms
+        launches += nl
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    be.close()
+    ms_step = (t1 - t0) * 1e3 / steps
+    return ms_step, stage_ms / steps, launches / steps
+
+
+def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, s_opts, y0, x0, size,
+                  gpu_maps):
+    """Oracle SHP counts and FP64 gamma/pairs on a size x size crop."""
+    n, wa, wr = 30, 21, 7
+    pr_c, pc_c = size + wa - 1, size + wr - 1
+    slc3 = slc.reshape(-1, pcols, n * 2)[y0:y0 + pr_c, x0:x0 + pc_c]
+    amp3 = amp.reshape(-1, pcols, n)[y0:y0 + pr_c, x0:x0 + pc_c]
+    val3 = valid.reshape(-1, pcols)[y0:y0 + pr_c, x0:x0 + pc_c]
+    so = cref.settings(n, **s_opts)
+    _, ocount, _, mask, _ = cref.run(so, pc_c, slc3, amp3, val3, pr_c, size, size,
+                                     nthreads=min(16, os.cpu_count() or 1))
+    c64, p64 = fp64.run(np.ascontiguousarray(slc3, np.float32), ocount, mask,
+                        cref.settings_dict(so), pc_c, size, size)
+    out = {}
+    for form, (coh, count, pairs) in gpu_maps.items():
+        cc = coh[y0:y0 + size, x0:x0 + size].reshape(-1)
+        kc = count[y0:y0 + size, x0:x0 + size].reshape(-1)
+        pc = pairs[:, y0:y0 + size, x0:x0 + size].reshape(pairs.shape[0], -1).T
+        d = np.abs(cc - c64)
+        dp = np.abs(pc - p64)
+        out[form] = {
+            "max_abs_dgamma": float(np.nanmax(d)),
+            "rms_dgamma": float(np.sqrt(np.nanmean(d ** 2))),
+            "pixels_over_1e-4": int(np.sum(d > 1e-4)),
+            "max_abs_dpair": float(np.nanmax(dp)),
+            "nan_mask_equal": bool(np.array_equal(np.isnan(cc), np.isnan(c64))),
+            "shp_count_mismatches": int(np.sum(kc != ocount)),
+        }
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="f16", choices=["f16", "bf16"])
+    ap.add_argument("--rows", type=int, default=2048)
+    ap.add_argument("--cols", type=int, default=2048)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=384)
+    ap.add_argument("--crop", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every output pixel is independent; rows would shard with a "
+              "20-row halo). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+    from oracle import cref, fp64
+
+    tcoh = ge.load_package()
+    n, rows, cols = 30, args.rows, args.cols
+    wa, wr = 21, 7
+    ha, hr = wa // 2, wr // 2
+    s_opts = dict(window=(wa, wr), shp_test="ks", alpha=0.05, min_shp=30,
+                  estimator="evd", bias=True, pairs="consecutive")
+    s = tcoh.make_settings(n, **s_opts)
+
+    t0 = time.perf_counter()
+    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0)
+    stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
+    slc, amp, valid = prepare_on_device(torch, stack, ha, hr)
+    del stack
+    pcols = cols + 2 * hr
+    coh = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
+    count = torch.empty((rows, cols), dtype=torch.int32, device="cuda")
+    pairs = torch.empty((s.npairs, rows, cols), dtype=torch.float32, device="cuda")
+    bufs = (slc, amp, valid, coh, count, pairs)
+    torch.cuda.synchronize()
+    log("scene %dx%d N=%d ready in %.1f s" % (rows, cols, n, time.perf_counter() - t0))
+
+    forms = {}
+    maps = {}
+    order = [args.precision] + ([] if args.no_plain else ["plain"])
+    for form in order:
+        steps = args.steps if form == args.precision else max(2, args.steps // 2)
+        ms, st_ms, nl = run_form(tcoh, s, pcols, bufs, rows, cols, form, steps,
+                                 args.warmup, torch)
+        forms[form] = (ms, st_ms, nl, steps)
+        maps[form] = (coh.cpu().numpy(), count.cpu().numpy(), pairs.cpu().numpy())
+        log("%s: %.2f ms/step, stages prep/shp/cov/eig ms %s" %
+            (form, ms, np.round(st_ms, 3).tolist()))
+
+    ms, st_ms, nl, steps = forms[args.precision]
+    npix = rows * cols
+    value = npix / (ms * 1e-3) / 1e6
+    count_h = maps[args.precision][1]
+    est = int(np.sum(count_h >= s.min_looks))
+    looks = count_h[count_h >= s.min_looks].astype(np.float64)
+
+    # Dominant kernel of the timed form and its algorithmic FLOP rate.
+    stage_names = ["prep", "shp", "cov", "eig"]
+    dom = int(np.argmax(st_ms))
+    if dom == 3:
+        flop = est * flops_eigen(n)
+        peak, kname = PEAK_FP32, "k_eig (FP32 vector ALU)"
+    elif dom == 2:
+        flop = float(np.sum(4.0 * n * (n + 1) * looks))
+        if args.precision == "plain":
+            peak, kname = PEAK_FP32, "k_cov_plain (FP32 vector ALU)"
+        else:
+            # three hi/lo products on the matrix cores
+            flop *= 3.0
+            peak, kname = PEAK_F16_MFMA, "k_cov_mfma (split-16-bit MFMA, 3 products)"
+    else:
+        flop, peak, kname = 0.0, PEAK_FP32, "k_" + stage_names[dom]
+    avg_launch_s = st_ms[dom] * 1e-3 / max(1.0, nl[dom])
+    achieved = flop / max(1.0, nl[dom]) / avg_launch_s if flop else 0.0
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,
+                "launches_per_step": float(nl[dom])}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 hi/lo split, f32 accumulate" if args.precision == "f16"
+                 else "bf16 hi/lo split, f32 accumulate",
+        "data": "synthetic",
+        "config": {"workload": "N=30 dates, %dx%d px, 21x7 window, KS alpha 0.05, EVD, "
+                               "min_shp 30, bias correction, consecutive pairs; "
+                               "Voronoi decorrelation scene seed 0" % (rows, cols),
+                   "estimated_pixels": est, "mean_shp": float(looks.mean()),
+                   "stage_ms": dict(zip(stage_names, np.round(st_ms, 3).tolist()))},
+        "roofline": roofline,
+    }
+    if "plain" in forms:
+        pms, pst, _, psteps = forms["plain"]
+        out["plain"] = {"value": npix / (pms * 1e-3) / 1e6, "ms_per_step": pms,
+                        "steps": psteps,
+                        "stage_ms": dict(zip(stage_names, np.round(pst, 3).tolist()))}
+        out["speedup_vs_plain"] = pms / ms
+
+    if args.crop > 0:
+        t0 = time.perf_counter()
+        y0 = max(0, rows // 2 - args.crop // 2)
+        x0 = max(0, cols // 2 - args.crop // 2)
+        size = min(args.crop, rows, cols)
+        out["accuracy"] = crop_accuracy(tcoh, cref, fp64, slc.cpu().numpy(),
+                                        amp.cpu().numpy(), valid.cpu().numpy(), pcols,
+                                        s_opts, y0, x0, size, maps)
+        out["accuracy"]["crop"] = [y0, x0, size]
+        log("accuracy crop in %.1f s" % (time.perf_counter() - t0))
+
+    if not args.no_cpu:
+        band = min(args.cpu_rows, rows)
+        threads = min(16, os.cpu_count() or 1)
+        so = cref.settings(n, **s_opts)
+        pr_b = band + wa - 1
+        sl = slc[:pr_b].cpu().numpy()
+        am = amp[:pr_b].cpu().numpy()
+        va = valid[:pr_b].cpu().numpy()
+        t0 = time.perf_counter()
+        cref.run(so, pcols, sl, am, va, pr_b, band, cols, with_mask=False,
+                 nthreads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": band * cols / dt / 1e6, "unit": "Mpixel/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d x %d px band of the same frame (%.1f s), FP32 C restatement "
+                      "of the reference kernel, OpenMP over rows" % (band, cols, dt)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

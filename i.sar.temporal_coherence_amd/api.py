@@ -1,0 +1,339 @@
+"""Python binding of libtcoh_hip.so, mirroring the module's interface.
+
+The reference is a GRASS module (main.c) calling its OpenCL backend through
+ocl_open / ocl_run / ocl_close (local_proto.h:104-112). This binding calls
+the same entry points of the HIP backend:
+
+  Settings / make_settings  struct settings and its derivation from the
+                            module options (main.c:553-608, stats.c:61-95)
+  Backend                   one ocl_open handle; run() = ocl_run on host
+                            arrays, run_device() = tcoh_run_device on
+                            device pointers (e.g. torch tensors)
+  temporal_coherence        the module's strip loop (main.c:718-825) over a
+                            planar complex stack, with read_strip's
+                            preparation (main.c:52-127): returns the
+                            coherence, SHP-count and pair-coherence maps
+
+Errors raise RuntimeError (the library is put in its returning error mode)
+instead of ending the process as G_fatal_error does. There is no CPU path:
+without the built library or a HIP device every call fails.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtcoh_hip.so")
+
+SHP_TESTS = {"ks": 0, "ad": 1, "tlog": 2}
+PRECISIONS = {"plain": 0, "f16": 1, "bf16": 2}
+
+
+class Settings(C.Structure):
+    """struct settings, local_proto.h:33-49."""
+
+    _fields_ = [
+        ("ndates", C.c_int),
+        ("win_az", C.c_int),
+        ("win_rg", C.c_int),
+        ("test", C.c_int),
+        ("alpha", C.c_double),
+        ("emi", C.c_int),
+        ("bias", C.c_int),
+        ("min_looks", C.c_int),
+        ("ks_max", C.c_int),
+        ("ad_norm", C.c_double),
+        ("ad_sigma", C.c_double),
+        ("ad_crit", C.c_double),
+        ("tlog_crit", C.c_double),
+        ("npairs", C.c_int),
+        ("pairs_all", C.c_int),
+        ("phase_reference", C.c_char_p),
+        ("orbit", C.c_char_p),
+    ]
+
+
+class Orbit(C.Structure):
+    """struct orbit, local_proto.h:52-57."""
+
+    _fields_ = [
+        ("n", C.c_int),
+        ("t", C.POINTER(C.c_double)),
+        ("pos", C.POINTER(C.c_double)),
+        ("vel", C.POINTER(C.c_double)),
+        ("source", C.c_char * 32),
+    ]
+
+
+class Geometry(C.Structure):
+    """struct geometry, local_proto.h:61-68."""
+
+    _fields_ = [
+        ("ndates", C.c_int),
+        ("grid_rows", C.c_int),
+        ("grid_cols", C.c_int),
+        ("step", C.c_int),
+        ("nheights", C.c_int),
+        ("heights", C.c_double * 3),
+        ("phase", C.POINTER(C.c_double)),
+        ("h0", C.c_double),
+    ]
+
+
+class Scene(C.Structure):
+    """struct tcoh_scene (include/tcoh_hip.h)."""
+
+    _fields_ = [
+        ("t_first", C.c_double),
+        ("dt", C.c_double),
+        ("tau0", C.c_double),
+        ("fs", C.c_double),
+        ("lambda_", C.c_double),
+        ("line0", C.c_double),
+        ("sample0", C.c_double),
+        ("rows", C.c_int),
+        ("cols", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libtcoh_hip.so (RuntimeError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libtcoh_hip.so is not built: run make -C i.sar.temporal_coherence_amd "
+            "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.shp_constants.argtypes = [C.POINTER(Settings)]
+    L.ocl_list_devices.argtypes = []
+    L.ocl_open.argtypes = [C.c_int, C.c_int, C.POINTER(Settings), C.c_int,
+                           C.POINTER(C.c_size_t)]
+    L.ocl_open.restype = vp
+    L.ocl_run.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+    L.ocl_close.argtypes = [vp]
+    L.tcoh_run_device.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp,
+                                  vp, vp, vp]
+    L.tcoh_set_precision.argtypes = [vp, C.c_int]
+    L.tcoh_set_timing.argtypes = [vp, C.c_int]
+    L.tcoh_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
+    L.tcoh_stage_launches.argtypes = [vp, C.POINTER(C.c_int)]
+    L.tcoh_set_error_mode.argtypes = [C.c_int]
+    L.tcoh_last_error.restype = C.c_char_p
+    L.tcoh_prepare_strip.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
+                                     C.c_int, C.c_int, C.c_int, vp, vp, vp]
+    L.orbit_state.argtypes = [C.POINTER(Orbit), C.c_double, vp, vp]
+    L.tcoh_geometry_init.argtypes = [C.POINTER(Geometry), C.POINTER(Scene),
+                                     C.c_int, C.c_int, C.POINTER(Orbit),
+                                     C.c_double, C.c_double, C.c_double]
+    L.geometry_phases.argtypes = [C.POINTER(Geometry), C.c_int, C.c_int,
+                                  C.c_double, vp]
+    L.geometry_free.argtypes = [C.POINTER(Geometry)]
+    L.tcoh_set_error_mode(1)
+    _lib = L
+    return L
+
+
+def _raise_if_error():
+    msg = lib().tcoh_last_error()
+    if msg:
+        raise RuntimeError(msg.decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def make_settings(n, window=(21, 7), shp_test="ks", alpha=0.05, min_shp=20,
+                  estimator="evd", bias=False, pairs=None):
+    """Settings from the module options (main.c:553-608).
+
+    window: (azimuth, range) odd sizes; pairs: None, "consecutive" or "all".
+    """
+    wa, wr = window
+    if wa < 1 or wr < 1 or wa % 2 == 0 or wr % 2 == 0:
+        raise ValueError("Window sizes must be odd, got %d,%d" % (wa, wr))
+    if not 0 < alpha < 1:
+        raise ValueError("Option alpha must be between 0 and 1")
+    s = Settings()
+    s.ndates = n
+    s.win_az, s.win_rg = wa, wr
+    s.test = SHP_TESTS[shp_test]
+    s.alpha = alpha
+    s.emi = 1 if estimator == "emi" else 0
+    s.bias = 1 if bias else 0
+    s.min_looks = max(min_shp, n)
+    if wa * wr < s.min_looks:
+        raise ValueError("The %dx%d window holds fewer pixels than the %d SHPs "
+                         "needed" % (wa, wr, s.min_looks))
+    s.pairs_all = 1 if pairs == "all" else 0
+    s.npairs = 0 if pairs is None else (n * (n - 1) // 2 if pairs == "all" else n - 1)
+    if shp_test == "ad" and not 0.001 <= alpha <= 0.25:
+        raise ValueError("Anderson-Darling test: alpha must be between 0.001 and 0.25")
+    lib().shp_constants(C.byref(s))
+    return s
+
+
+def pair_list(n, mode):
+    """Date pairs in output order (tcoh.cl:19-23; main.c:610-615)."""
+    if mode == "all":
+        return [(i, j) for i in range(n - 1) for j in range(i + 1, n)]
+    return [(i, i + 1) for i in range(n - 1)]
+
+
+class Backend:
+    """One ocl_open handle (ocl.c:191-254)."""
+
+    def __init__(self, settings, padded_cols, platform=-1, device=-1,
+                 precision=None):
+        self.settings = settings
+        self.padded_cols = padded_cols
+        ma = C.c_size_t(0)
+        self.h = lib().ocl_open(platform, device, C.byref(settings), padded_cols,
+                                C.byref(ma))
+        _raise_if_error()
+        if not self.h:
+            raise RuntimeError("ocl_open failed")
+        self.max_alloc = ma.value
+        if precision is not None:
+            self.set_precision(precision)
+
+    def set_precision(self, precision):
+        lib().tcoh_set_precision(self.h, PRECISIONS.get(precision, precision))
+        _raise_if_error()
+
+    def set_timing(self, on):
+        lib().tcoh_set_timing(self.h, 1 if on else 0)
+
+    def stage_times(self):
+        """Last run's (prep, shp, covariance, eigen) device ms and launch
+        counts (timing must be on)."""
+        ms = (C.c_double * 4)()
+        nl = (C.c_int * 4)()
+        lib().tcoh_stage_times(self.h, ms)
+        lib().tcoh_stage_launches(self.h, nl)
+        return list(ms), list(nl)
+
+    def run(self, slc, amp, valid, padded_rows, rows, cols):
+        """ocl_run on host arrays: (coh, count, pairs (npairs, rows*cols))."""
+        s = self.settings
+        slc = np.ascontiguousarray(slc, np.float32)
+        amp = np.ascontiguousarray(amp, np.float32)
+        valid = np.ascontiguousarray(valid, np.uint8)
+        need = padded_rows * self.padded_cols
+        if valid.size < need or amp.size < need * s.ndates or \
+                slc.size < need * s.ndates * 2:
+            raise ValueError("strip buffers smaller than padded_rows x padded_cols")
+        coh = np.empty(rows * cols, np.float32)
+        count = np.empty(rows * cols, np.int32)
+        pairs = np.empty((s.npairs, rows * cols), np.float32) if s.npairs else None
+        lib().ocl_run(self.h, _ptr(slc), _ptr(amp), _ptr(valid), padded_rows, rows,
+                      cols, _ptr(coh), _ptr(count), _ptr(pairs))
+        _raise_if_error()
+        return coh, count, pairs
+
+    def run_device(self, slc_ptr, amp_ptr, valid_ptr, padded_rows, rows, cols,
+                   coh_ptr, count_ptr, pairs_ptr=None, stream=None):
+        """tcoh_run_device on device pointers (ints), asynchronous."""
+        lib().tcoh_run_device(self.h, slc_ptr, amp_ptr, valid_ptr, padded_rows,
+                              rows, cols, coh_ptr, count_ptr, pairs_ptr, stream)
+        _raise_if_error()
+
+    def close(self):
+        if self.h:
+            lib().ocl_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prepare_strip(i, q, row0, padded_rows, halo_cols, phase=None, elev_null=None):
+    """read_strip (main.c:52-127) on planar float32 (n, nrows, ncols) I/Q."""
+    i = np.ascontiguousarray(i, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    n, nrows, ncols = i.shape
+    pc = ncols + 2 * halo_cols
+    slc = np.empty(padded_rows * pc * n * 2, np.float32)
+    amp = np.empty(padded_rows * pc * n, np.float32)
+    valid = np.empty(padded_rows * pc, np.uint8)
+    ph = None if phase is None else np.ascontiguousarray(phase, np.float64)
+    en = None if elev_null is None else np.ascontiguousarray(elev_null, np.uint8)
+    lib().tcoh_prepare_strip(n, nrows, ncols, _ptr(i), _ptr(q), _ptr(ph), _ptr(en),
+                             row0, padded_rows, halo_cols, _ptr(slc), _ptr(amp),
+                             _ptr(valid))
+    return slc, amp, valid
+
+
+def strip_rows_for(settings, ncols, memory_mb, max_alloc, nrows):
+    """Strip height of main.c:724-750."""
+    n, wa, wr = settings.ndates, settings.win_az, settings.win_rg
+    padded_cols = ncols + wr - 1
+    per_row = padded_cols * (12 * n + 1) + ncols * 4 * (2 + settings.npairs)
+    rows = int(memory_mb * 1024 * 1024 // per_row) - (wa - 1)
+    if max_alloc > 0:
+        dev_rows = int(max_alloc // (padded_cols * n * 8)) - (wa - 1)
+        rows = min(rows, dev_rows)
+        if settings.npairs:
+            rows = min(rows, int(max_alloc // (ncols * settings.npairs * 4)))
+    if rows < 1:
+        raise RuntimeError("Not enough memory for one row of %d columns and %d "
+                           "dates; increase memory or reduce the region" % (ncols, n))
+    return min(rows, nrows)
+
+
+def temporal_coherence(stack, window=(21, 7), shp_test="ks", alpha=0.05,
+                       min_shp=20, estimator="evd", bias=False, pairs=None,
+                       phase=None, elev_null=None, memory=300, precision=None,
+                       platform=-1, device=-1):
+    """The module's estimate over a complex stack (N, rows, cols).
+
+    NaN samples are nulls. phase (N, rows, cols) is the geometric phase
+    removed from every date (read_strip's rotation). Returns coh (rows,
+    cols) float32 with NaN where not estimated, count (rows, cols) int32
+    with -1 for invalid centres, and pairs (npairs, rows, cols) or None.
+    """
+    stack = np.asarray(stack)
+    n, nrows, ncols = stack.shape
+    s = make_settings(n, window, shp_test, alpha, min_shp, estimator, bias, pairs)
+    ha, hr = s.win_az // 2, s.win_rg // 2
+    i = np.ascontiguousarray(stack.real, np.float32)
+    q = np.ascontiguousarray(stack.imag, np.float32)
+    coh = np.empty((nrows, ncols), np.float32)
+    count = np.empty((nrows, ncols), np.int32)
+    pr = np.empty((s.npairs, nrows, ncols), np.float32) if s.npairs else None
+    with Backend(s, ncols + 2 * hr, platform, device, precision) as be:
+        strip = strip_rows_for(s, ncols, memory, be.max_alloc, nrows)
+        for row0 in range(0, nrows, strip):
+            rows = min(strip, nrows - row0)
+            prow = rows + 2 * ha
+            slc, amp, valid = prepare_strip(i, q, row0 - ha, prow, hr, phase,
+                                            elev_null)
+            c, k, p = be.run(slc, amp, valid, prow, rows, ncols)
+            coh[row0:row0 + rows] = c.reshape(rows, ncols)
+            count[row0:row0 + rows] = k.reshape(rows, ncols)
+            if pr is not None:
+                pr[:, row0:row0 + rows] = p.reshape(s.npairs, rows, ncols)
+    return coh, count, pr
+
+
+def list_devices():
+    """ocl_list_devices (ocl.c:105-136), printed to stdout."""
+    lib().ocl_list_devices()
+    _raise_if_error()

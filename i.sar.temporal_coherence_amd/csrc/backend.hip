@@ -1,0 +1,596 @@
+/* HIP backend of the phase-linking module: the ocl_* entry points of
+ * local_proto.h:104-112 (ocl.c:105-391 in the reference) and the
+ * extensions declared in include/tcoh_hip.h.
+ *
+ * One handle = one device, one in-order HIP stream and the settings fixed
+ * at open (ocl.c:191-254). A strip runs as
+ *   prep      amplitudes to date-planar order (+ log statistics for tlog)
+ *   shp       SHP mask and count per output pixel
+ *   [split]   hi/lo 16-bit copy of the samples (mixed precision only)
+ *   per batch of output pixels (bounded scratch):
+ *     cov     coherence matrix T and pair coherences
+ *     eig     phase linking and temporal coherence
+ * Buffers only grow (ocl.c:264-277). The reference splits its launches
+ * into ~0.5 s chunks against GPU watchdogs (ocl.c:329-363); here the
+ * batches bound every launch instead.
+ */
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "../../include/tcoh_hip.h"
+
+namespace {
+
+int g_error_mode = 0;
+thread_local std::string g_last_error;
+
+struct tcoh_error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+[[noreturn]] void fail(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    throw tcoh_error(buf);
+}
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        fail("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+}
+
+/* Report an error as the reference does (G_fatal_error: message, exit)
+   or, in error mode 1, keep it for tcoh_last_error(). */
+void report(const char *msg)
+{
+    g_last_error = msg;
+    if (g_error_mode == 0) {
+        fprintf(stderr, "ERROR: %s\n", msg);
+        fflush(stderr);
+        exit(EXIT_FAILURE);
+    }
+}
+
+template <class F> auto guarded(F f, decltype(f()) on_error) -> decltype(f())
+{
+    g_last_error.clear();
+    try {
+        return f();
+    }
+    catch (const std::exception &e) {
+        report(e.what());
+        return on_error;
+    }
+}
+
+template <class F> void guarded_void(F f)
+{
+    g_last_error.clear();
+    try {
+        f();
+    }
+    catch (const std::exception &e) {
+        report(e.what());
+    }
+}
+
+/* The float a kernel compiled with "-DX=%.9ef" holds (ocl.c:222-229). */
+float kernel_float(double x)
+{
+    char buf[64];
+
+    snprintf(buf, sizeof(buf), "%.9e", x);
+    return strtof(buf, nullptr);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t size = 0;
+
+    void ensure(size_t bytes, const char *what)
+    {
+        if (p && size >= bytes)
+            return;
+        if (p)
+            check(hipFree(p), "hipFree");
+        p = nullptr;
+        size = 0;
+        check(hipMalloc(&p, bytes ? bytes : 1), what);
+        size = bytes;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        size = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+/* Largest T scratch of one batch. */
+const size_t TBUF_BYTES = (size_t)512 << 20;
+
+} // namespace
+
+/* Stage timing: one event pair around every kernel launch of a run. */
+struct StageTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> stage;
+    size_t used = 0;
+
+    void begin() { used = 0; stage.clear(); }
+    hipEvent_t next()
+    {
+        if (used == ev.size()) {
+            hipEvent_t e;
+
+            check(hipEventCreate(&e), "hipEventCreate");
+            ev.push_back(e);
+        }
+        return ev[used++];
+    }
+    void mark(int st, hipStream_t s, bool start)
+    {
+        check(hipEventRecord(next(), s), "hipEventRecord");
+        if (start)
+            stage.push_back(st);
+    }
+    void release()
+    {
+        for (hipEvent_t e : ev)
+            (void)hipEventDestroy(e);
+        ev.clear();
+    }
+};
+
+struct ocl {
+    int device;
+    hipStream_t stream;
+    struct settings s;
+    int padded_cols;
+    int precision;
+    int timing;
+    double stage_ms[4];
+    int stage_launches[4];
+    StageTimer timer;
+    /* ocl_run's transfer buffers */
+    DevBuf slc, amp, valid, coh, count, pairs;
+    /* scratch */
+    DevBuf ampT, lstat, mask, adtab, tbuf, hl, absmax;
+    int adtab_n;
+};
+
+static int env_precision()
+{
+    const char *e = getenv("TCOH_PRECISION");
+
+    if (!e || !*e)
+        return TCOH_MIXED_F16;
+    if (!strcmp(e, "plain"))
+        return TCOH_PLAIN;
+    if (!strcmp(e, "bf16"))
+        return TCOH_MIXED_BF16;
+    return TCOH_MIXED_F16;
+}
+
+static bool eig_supported(int n);
+static hipError_t launch_eig(const KParams &kp, const float2 *tbuf,
+                             const int *count, float *coh, int pix0, int npix,
+                             hipStream_t s);
+
+extern "C" void ocl_list_devices(void)
+{
+    guarded_void([] {
+        int nd = 0;
+
+        check(hipGetDeviceCount(&nd), "hipGetDeviceCount");
+        if (nd == 0)
+            fail("No HIP device found");
+        fprintf(stdout, "platform|device|type|name|version|compute_units|"
+                        "global_memory_mb\n");
+        for (int d = 0; d < nd; d++) {
+            hipDeviceProp_t pr;
+            int rt = 0;
+
+            check(hipGetDeviceProperties(&pr, d), "hipGetDeviceProperties");
+            (void)hipRuntimeGetVersion(&rt);
+            fprintf(stdout, "0|%d|GPU|%s (%s)|HIP %d.%d|%d|%lu\n", d, pr.name,
+                    pr.gcnArchName, rt / 10000000, (rt / 100000) % 100,
+                    pr.multiProcessorCount,
+                    (unsigned long)(pr.totalGlobalMem / (1024 * 1024)));
+        }
+    });
+}
+
+extern "C" struct ocl *ocl_open(int platform, int device,
+                                const struct settings *s, int padded_cols,
+                                size_t *max_alloc)
+{
+    return guarded(
+        [&]() -> struct ocl * {
+            int nd = 0;
+
+            check(hipGetDeviceCount(&nd), "hipGetDeviceCount");
+            if (platform >= 0 && platform != 0)
+                fail("Platform %d not found (1 platform, see -l)", platform);
+            if (platform < 0 && device >= 0)
+                fail("Option device requires option platform");
+            if (nd == 0)
+                fail("No HIP device found");
+            if (device < 0)
+                device = 0;
+            if (device >= nd)
+                fail("Device %d not found on platform 0 (%d devices, see -l)",
+                     device, nd);
+            if (s->ndates < 3 || s->ndates > MAX_DATES)
+                fail("The stack must have 3 to %d dates, not %d", MAX_DATES,
+                     s->ndates);
+            if (!eig_supported(s->ndates))
+                fail("No kernel built for %d dates", s->ndates);
+            if (s->win_az < 1 || s->win_rg < 1 || s->win_az % 2 == 0 ||
+                s->win_rg % 2 == 0)
+                fail("Window sizes must be odd, got %d,%d", s->win_az,
+                     s->win_rg);
+            if (s->win_az * s->win_rg > TCOH_MAX_WINDOW)
+                fail("Windows of more than %d pixels are not supported",
+                     TCOH_MAX_WINDOW);
+            if (padded_cols < s->win_rg)
+                fail("Padded strip width %d below the window width %d",
+                     padded_cols, s->win_rg);
+
+            struct ocl *o = new struct ocl();
+            hipDeviceProp_t pr;
+            size_t fr = 0, tot = 0;
+
+            o->device = device;
+            o->s = *s;
+            o->padded_cols = padded_cols;
+            o->precision = env_precision();
+            o->timing = 0;
+            o->adtab_n = 0;
+            check(hipSetDevice(device), "hipSetDevice");
+            check(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking),
+                  "hipStreamCreate");
+            check(hipGetDeviceProperties(&pr, device), "hipGetDeviceProperties");
+            check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+            /* The strip's samples (8N bytes a pixel) take about 40 % of
+               what one strip needs on the device (amplitudes twice,
+               masks, outputs, T scratch). */
+            if (max_alloc)
+                *max_alloc = (size_t)((double)fr * 0.4);
+            return o;
+        },
+        nullptr);
+}
+
+extern "C" void tcoh_set_precision(struct ocl *o, int precision)
+{
+    guarded_void([&] {
+        if (precision < TCOH_PLAIN || precision > TCOH_MIXED_BF16)
+            fail("Unknown precision %d", precision);
+        o->precision = precision;
+    });
+}
+
+extern "C" void tcoh_set_timing(struct ocl *o, int on) { o->timing = on; }
+
+extern "C" void tcoh_stage_times(const struct ocl *o, double ms[4])
+{
+    for (int i = 0; i < 4; i++)
+        ms[i] = o->stage_ms[i];
+}
+
+extern "C" void tcoh_stage_launches(const struct ocl *o, int launches[4])
+{
+    for (int i = 0; i < 4; i++)
+        launches[i] = o->stage_launches[i];
+}
+
+extern "C" void tcoh_set_error_mode(int mode) { g_error_mode = mode; }
+
+extern "C" const char *tcoh_last_error(void) { return g_last_error.c_str(); }
+
+/* q(j, m) = (float)(2Nm - jN)^2 / (float)(j(2N - j)) with the two roundings
+   of tcoh.cl:71-72. */
+#pragma clang fp contract(off)
+static void fill_adtab(int n, float *q)
+{
+    for (int j = 0; j < 2 * n; j++)
+        for (int m = 0; m <= n; m++) {
+            if (j == 0) {
+                q[m] = 0.0f;
+                continue;
+            }
+            const float num = (float)(2 * n * m - j * n);
+            const float den = (float)(j * (2 * n - j));
+            volatile float sq = num * num;
+
+            q[j * (n + 1) + m] = sq / den;
+        }
+}
+
+static void run_device(struct ocl *o, const void *slc, const void *amp,
+                       const void *valid, int padded_rows, int rows, int cols,
+                       void *coh, void *count, void *pairs, hipStream_t st)
+{
+    const struct settings &s = o->s;
+    KParams kp;
+
+    if (rows <= 0 || cols <= 0)
+        return;
+    if (cols + s.win_rg - 1 != o->padded_cols)
+        fail("Strip of %d columns does not match the padded width %d", cols,
+             o->padded_cols);
+    if (padded_rows < rows + s.win_az - 1)
+        fail("Padded strip of %d rows too small for %d rows", padded_rows,
+             rows);
+    if ((size_t)padded_rows * o->padded_cols > (size_t)INT32_MAX)
+        fail("Strip of %d x %d pixels is too large", padded_rows,
+             o->padded_cols);
+    if (s.npairs && !pairs)
+        fail("Pair coherences requested without an output buffer");
+    check(hipSetDevice(o->device), "hipSetDevice");
+
+    kp.n = s.ndates;
+    kp.wa = s.win_az;
+    kp.wr = s.win_rg;
+    kp.w = o->padded_cols;
+    kp.rows = rows;
+    kp.cols = cols;
+    kp.npad = padded_rows * o->padded_cols;
+    kp.test = (int)s.test;
+    kp.emi = s.emi;
+    kp.bias = s.bias;
+    kp.min_looks = s.min_looks;
+    kp.ks_max = s.ks_max;
+    kp.npairs = s.npairs;
+    kp.pairs_all = s.pairs_all;
+    kp.words = (s.win_az * s.win_rg + 31) / 32;
+    kp.ad_norm = kernel_float(s.ad_norm);
+    kp.ad_sigma = kernel_float(s.ad_sigma);
+    kp.ad_crit = kernel_float(s.ad_crit);
+    kp.tlog_crit = kernel_float(s.tlog_crit);
+
+    const size_t npad = (size_t)kp.npad, nout = (size_t)rows * cols;
+    const int n = kp.n, ntri = n * (n - 1) / 2;
+
+    o->ampT.ensure(npad * n * sizeof(float), "hipMalloc(ampT)");
+    o->mask.ensure(nout * kp.words * sizeof(uint32_t), "hipMalloc(mask)");
+    if (kp.test == 2)
+        o->lstat.ensure(npad * sizeof(float2), "hipMalloc(lstat)");
+    if (kp.test == 1 && o->adtab_n != n) {
+        float *q = (float *)malloc(sizeof(float) * 2 * n * (n + 1));
+
+        fill_adtab(n, q);
+        o->adtab.ensure(sizeof(float) * 2 * n * (n + 1), "hipMalloc(adtab)");
+        check(hipMemcpy(o->adtab.p, q, sizeof(float) * 2 * n * (n + 1),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy(adtab)");
+        free(q);
+        o->adtab_n = n;
+    }
+    size_t batch = TBUF_BYTES / ((size_t)ntri * sizeof(float2));
+    if (batch > nout)
+        batch = nout;
+    batch = (batch + 63) / 64 * 64;
+    o->tbuf.ensure(batch * ntri * sizeof(float2), "hipMalloc(tbuf)");
+
+    StageTimer &tm = o->timer;
+    const bool timing = o->timing != 0;
+    auto mark = [&](int stage, bool start) {
+        if (timing)
+            tm.mark(stage, st, start);
+    };
+
+    if (timing)
+        tm.begin();
+    mark(0, true);
+    check(launch_prep(kp, (const float *)amp, o->ampT.as<float>(),
+                      (const unsigned char *)valid, o->lstat.as<float2>(), st),
+          "prep kernel");
+    mark(0, false);
+    mark(1, true);
+    check(launch_shp(kp, o->ampT.as<float>(), (const unsigned char *)valid,
+                     o->lstat.as<float2>(), o->adtab.as<float>(),
+                     o->mask.as<uint32_t>(), (int *)count, st),
+          "shp kernel");
+    mark(1, false);
+
+    const bool mixed = o->precision != TCOH_PLAIN;
+    if (mixed) {
+        /* Samples as (hi, lo) 16-bit pairs, one zero pixel appended for
+           the padding of candidate lists. Power-of-two scale: T is
+           invariant to a common factor of the samples. */
+        const int bf16 = o->precision == TCOH_MIXED_BF16;
+        float scale = 1.0f;
+
+        o->hl.ensure((npad + 1) * n * 2 * sizeof(uint32_t), "hipMalloc(hl)");
+        if (!bf16) {
+            unsigned int bits = 0;
+
+            o->absmax.ensure(sizeof(unsigned int), "hipMalloc(absmax)");
+            check(launch_absmax((const float2 *)slc, npad * n,
+                                o->absmax.as<unsigned int>(), st),
+                  "absmax kernel");
+            check(hipMemcpyAsync(&bits, o->absmax.p, sizeof(bits),
+                                 hipMemcpyDeviceToHost, st),
+                  "hipMemcpy(absmax)");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            float mx;
+            memcpy(&mx, &bits, sizeof(mx));
+            if (mx > 0.0f && isfinite(mx)) {
+                int ex;
+
+                frexpf(mx, &ex);
+                /* largest magnitude in [2^13, 2^14): hi fits f16, products
+                   and sums of up to 1024 candidates stay far below FP32
+                   overflow */
+                scale = ldexpf(1.0f, 14 - ex);
+            }
+        }
+        /* counted with the preparation stage */
+        mark(0, true);
+        check(launch_split(kp, (const float2 *)slc, o->hl.as<uint32_t>(), bf16,
+                           scale, st),
+              "split kernel");
+        mark(0, false);
+    }
+
+    for (size_t p0 = 0; p0 < nout; p0 += batch) {
+        const int np = (int)(p0 + batch <= nout ? batch : nout - p0);
+
+        mark(2, true);
+        if (mixed)
+            check(launch_cov_mfma(kp, o->hl.as<uint32_t>(),
+                                  o->mask.as<uint32_t>(), (const int *)count,
+                                  o->tbuf.as<float2>(), (float *)pairs,
+                                  (int)p0, np,
+                                  o->precision == TCOH_MIXED_BF16, st),
+                  "covariance kernel");
+        else
+            check(launch_cov_plain(kp, (const float2 *)slc,
+                                   o->mask.as<uint32_t>(), (const int *)count,
+                                   o->tbuf.as<float2>(), (float *)pairs,
+                                   (int)p0, np, st),
+                  "covariance kernel");
+        mark(2, false);
+        mark(3, true);
+        check(launch_eig(kp, o->tbuf.as<float2>(), (const int *)count,
+                         (float *)coh, (int)p0, np, st),
+              "eigen kernel");
+        mark(3, false);
+    }
+    if (timing) {
+        for (int i = 0; i < 4; i++) {
+            o->stage_ms[i] = 0.0;
+            o->stage_launches[i] = 0;
+        }
+        check(hipEventSynchronize(tm.ev[tm.used - 1]), "hipEventSynchronize");
+        for (size_t k = 0; k < tm.stage.size(); k++) {
+            float ms;
+
+            check(hipEventElapsedTime(&ms, tm.ev[2 * k], tm.ev[2 * k + 1]),
+                  "hipEventElapsedTime");
+            o->stage_ms[tm.stage[k]] += ms;
+            o->stage_launches[tm.stage[k]]++;
+        }
+    }
+}
+
+extern "C" void tcoh_run_device(struct ocl *o, const void *slc,
+                                const void *amp, const void *valid,
+                                int padded_rows, int rows, int cols, void *coh,
+                                void *count, void *pairs, void *stream)
+{
+    guarded_void([&] {
+        run_device(o, slc, amp, valid, padded_rows, rows, cols, coh, count,
+                   pairs, stream ? (hipStream_t)stream : o->stream);
+    });
+}
+
+extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
+                        const unsigned char *valid, int padded_rows, int rows,
+                        int cols, float *coh, int *count, float *pairs)
+{
+    guarded_void([&] {
+        const size_t npad = (size_t)padded_rows * o->padded_cols;
+        const size_t nout = (size_t)rows * cols;
+        const size_t n = o->s.ndates;
+        hipStream_t st = o->stream;
+
+        check(hipSetDevice(o->device), "hipSetDevice");
+        o->slc.ensure(npad * n * 2 * sizeof(float), "hipMalloc(slc)");
+        o->amp.ensure(npad * n * sizeof(float), "hipMalloc(amp)");
+        o->valid.ensure(npad, "hipMalloc(valid)");
+        o->coh.ensure(nout * sizeof(float), "hipMalloc(coh)");
+        o->count.ensure(nout * sizeof(int), "hipMalloc(count)");
+        o->pairs.ensure((o->s.npairs ? o->s.npairs * nout : 1) * sizeof(float),
+                        "hipMalloc(pairs)");
+        check(hipMemcpyAsync(o->slc.p, slc, npad * n * 2 * sizeof(float),
+                             hipMemcpyHostToDevice, st),
+              "hipMemcpy(slc)");
+        check(hipMemcpyAsync(o->amp.p, amp, npad * n * sizeof(float),
+                             hipMemcpyHostToDevice, st),
+              "hipMemcpy(amp)");
+        check(hipMemcpyAsync(o->valid.p, valid, npad, hipMemcpyHostToDevice,
+                             st),
+              "hipMemcpy(valid)");
+        run_device(o, o->slc.p, o->amp.p, o->valid.p, padded_rows, rows, cols,
+                   o->coh.p, o->count.p, o->pairs.p, st);
+        check(hipMemcpyAsync(coh, o->coh.p, nout * sizeof(float),
+                             hipMemcpyDeviceToHost, st),
+              "hipMemcpy(coh)");
+        if (o->s.npairs)
+            check(hipMemcpyAsync(pairs, o->pairs.p,
+                                 o->s.npairs * nout * sizeof(float),
+                                 hipMemcpyDeviceToHost, st),
+                  "hipMemcpy(pairs)");
+        check(hipMemcpyAsync(count, o->count.p, nout * sizeof(int),
+                             hipMemcpyDeviceToHost, st),
+              "hipMemcpy(count)");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+extern "C" void ocl_close(struct ocl *o)
+{
+    if (!o)
+        return;
+    guarded_void([&] {
+        (void)hipSetDevice(o->device);
+        (void)hipStreamSynchronize(o->stream);
+        DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
+                          &o->count, &o->pairs, &o->ampT,  &o->lstat,
+                          &o->mask, &o->adtab, &o->tbuf,  &o->hl,
+                          &o->absmax};
+        for (DevBuf *b : bufs)
+            b->release();
+        o->timer.release();
+        (void)hipStreamDestroy(o->stream);
+        delete o;
+    });
+}
+
+/* Stack depths with a compiled eigen kernel: TCOH_EIG_FOR_EACH(X) expands
+   to X(3) X(4) ... for the depths the build compiled. */
+#define TCOH_EIG_CASE(NN)                                                    \
+    case NN:                                                                 \
+        return launch_eig_n<NN>(kp, tbuf, count, coh, pix0, npix, s);
+
+static hipError_t launch_eig(const KParams &kp, const float2 *tbuf,
+                             const int *count, float *coh, int pix0, int npix,
+                             hipStream_t s)
+{
+    switch (kp.n) {
+        TCOH_EIG_FOR_EACH(TCOH_EIG_CASE)
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+#define TCOH_EIG_ITEM(NN) NN,
+static bool eig_supported(int n)
+{
+    static const int list[] = {TCOH_EIG_FOR_EACH(TCOH_EIG_ITEM)};
+
+    for (int v : list)
+        if (v == n)
+            return true;
+    return false;
+}

@@ -1,0 +1,176 @@
+/* C ABI of the MI355X (gfx950) phase-linking backend.
+ *
+ * Drop-in for the OpenCL backend of i.sar.temporal_coherence: the types and
+ * the four ocl_* entry points keep the layout, names and meaning of the
+ * reference's local_proto.h, so that main.c links against libtcoh_hip.so
+ * instead of ocl.c unchanged. Everything else below is an extension (marked
+ * as such) that the reference has no counterpart for.
+ *
+ * Errors: like the reference (G_fatal_error, ocl.c:44-48), every failure
+ * prints a message and ends the process, unless tcoh_set_error_mode(1) was
+ * called, in which case the failing entry point returns and tcoh_last_error()
+ * holds the message (used by bindings that must not exit).
+ */
+#ifndef TCOH_HIP_H
+#define TCOH_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* local_proto.h:9-10: largest supported stack depth. */
+#define MAX_DATES 64
+
+/* local_proto.h:12 */
+enum shp_test { SHP_KS = 0, SHP_AD = 1, SHP_TLOG = 2 };
+
+/* local_proto.h:33-49, field for field. */
+struct settings {
+    int ndates;
+    int win_az, win_rg;
+    enum shp_test test;
+    double alpha;
+    int emi;
+    int bias;
+    int min_looks;
+    int ks_max;
+    double ad_norm, ad_sigma, ad_crit;
+    double tlog_crit;
+    int npairs;    /* pair coherence maps to write, 0 for none */
+    int pairs_all; /* all pairs i < j instead of consecutive ones */
+    const char *phase_reference; /* none, ellipsoid or elevation */
+    const char *orbit;           /* orbit source, NULL without phase */
+};
+
+/* local_proto.h:52-57, field for field. */
+struct orbit {
+    int n;
+    double *t;
+    double (*pos)[3], (*vel)[3];
+    char source[32];
+};
+
+/* local_proto.h:61-68, field for field. */
+struct geometry {
+    int ndates;
+    int grid_rows, grid_cols, step;
+    int nheights;
+    double heights[3];
+    double *phase; /* [node][height][date] */
+    double h0;     /* height used without elevation map */
+};
+
+/* stats.c:61-95: KS_MAX, AD_NORM, AD_SIGMA, AD_CRIT, TLOG_CRIT from
+   s->ndates, s->alpha and s->test. */
+void shp_constants(struct settings *s);
+
+/* orbit.c:324-346: position and velocity at t by 8-point Lagrange
+   interpolation of the state vectors. */
+void orbit_state(const struct orbit *o, double t, double *pos, double *vel);
+
+/* ocl.c:105-136: print platform|device|type|name|version|compute_units|
+   global_memory_mb, one line per HIP device. */
+void ocl_list_devices(void);
+
+/* ocl.c:191-254: select the device (platform -1 or 0 is the HIP runtime;
+   device -1 is the first GPU), bind the settings (fixed for the life of the
+   handle) and the padded strip width, and report the largest allocation the
+   strip sizing of main.c:731-742 may use. */
+struct ocl;
+struct ocl *ocl_open(int platform, int device, const struct settings *s,
+                     int padded_cols, size_t *max_alloc);
+
+/* ocl.c:279-375: estimate one strip. slc is float[padded_rows][padded_cols]
+   [N][2], amp float[..][N] sorted increasingly, valid u8[..]; outputs are
+   row-major rows x cols: coh (NaN where not estimated), count (-1 for an
+   invalid centre) and, when s->npairs > 0, pairs planar [npairs][rows][cols]
+   (pairs may be NULL otherwise). Synchronous. */
+void ocl_run(struct ocl *o, const float *slc, const float *amp,
+             const unsigned char *valid, int padded_rows, int rows, int cols,
+             float *coh, int *count, float *pairs);
+
+/* ocl.c:377-391 */
+void ocl_close(struct ocl *o);
+
+/* ---- Flat-earth and topographic phase (geometry.c, FP64 on the GPU) ---- */
+
+/* The numbers geometry_init reads from the reference date's metadata
+   (geometry.c:166-197): first line time (s since 1970), azimuth time
+   interval, slant range time of the first sample, range sampling rate,
+   wavelength, and the line/sample of the first region cell; plus the region
+   size. Extension: the reference parses them from description.json. */
+struct tcoh_scene {
+    double t_first, dt, tau0, fs, lambda;
+    double line0, sample0;
+    int rows, cols;
+};
+
+/* geometry.c:161-254 with the metadata lookups replaced by scene: the phase
+   grid [node][height][date] on 16-pixel nodes, computed on the GPU in FP64
+   (geolocation Newton on the reference orbit, zero-Doppler Newton per date).
+   Orbits are host structs; n dates, ref the reference date. */
+void tcoh_geometry_init(struct geometry *g, const struct tcoh_scene *scene,
+                        int n, int ref, const struct orbit *orbits,
+                        double hmin, double hmax, double h0);
+
+/* geometry.c:256-291 */
+void geometry_phases(const struct geometry *g, int row, int col, double h,
+                     double *phase);
+
+/* geometry.c:293-296 */
+void geometry_free(struct geometry *g);
+
+/* ---- Extensions (no counterpart in the reference) ---- */
+
+/* Arithmetic of the covariance stage. PLAIN: FP32 FMA on the vector ALUs,
+   as the reference kernel. MIXED_F16 / MIXED_BF16: hi/lo split 16-bit
+   operands, three products on the matrix cores with FP32 accumulation. */
+enum tcoh_precision {
+    TCOH_PLAIN = 0,
+    TCOH_MIXED_F16 = 1,
+    TCOH_MIXED_BF16 = 2
+};
+
+/* Select the covariance arithmetic of a handle (default: the environment
+   variable TCOH_PRECISION = plain | f16 | bf16, else f16). */
+void tcoh_set_precision(struct ocl *o, int precision);
+
+/* main.c:52-127 (read_strip) without the GRASS row reads: build the padded
+   strip from planar complex dates. i, q are float[n][nrows][ncols] (NaN is
+   a null); rows row0 .. row0+padded_rows-1 of the region (outside: invalid),
+   halo_cols invalid columns on each side. With phase (double
+   [n][nrows][ncols], may be NULL) date k is multiplied by exp(i phase_k)
+   after the amplitudes are taken; elev_null (u8[nrows][ncols], may be NULL)
+   marks pixels without a height. Outputs as ocl_run's inputs. */
+void tcoh_prepare_strip(int n, int nrows, int ncols, const float *i,
+                        const float *q, const double *phase,
+                        const unsigned char *elev_null, int row0,
+                        int padded_rows, int halo_cols, float *slc,
+                        float *amp, unsigned char *valid);
+
+/* ocl_run on device-resident buffers (hipMalloc'd, same layouts), enqueued
+   on stream (a hipStream_t; NULL is the handle's own stream). Returns
+   without synchronising. */
+void tcoh_run_device(struct ocl *o, const void *slc, const void *amp,
+                     const void *valid, int padded_rows, int rows, int cols,
+                     void *coh, void *count, void *pairs, void *stream);
+
+/* Device time of the last run's stages in milliseconds (prep incl. the
+   hi/lo split, shp, covariance, eigen), summed over the stage's kernel
+   launches, measured with HIP events on the run's stream when timing is
+   on; and the number of launches of each stage. */
+void tcoh_set_timing(struct ocl *o, int on);
+void tcoh_stage_times(const struct ocl *o, double ms[4]);
+void tcoh_stage_launches(const struct ocl *o, int launches[4]);
+
+/* 0 (default): fatal errors end the process. 1: they return. */
+void tcoh_set_error_mode(int mode);
+const char *tcoh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,115 @@
+"""The C ABI library and its host code, without a GPU."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "tcoh_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    # "type name(" at a declaration; "double (*pos)[3]" is a field, not one
+    found = re.findall(r"\b(?:void|int|double|float|char|ocl)\s*\**\s*"
+                       r"([a-z_][a-z0-9_]*)\s*\(", text)
+    return sorted(set(found))
+
+
+def test_library_exports_every_declared_symbol(tcoh):
+    lib = tcoh.api.lib()
+    names = declared_functions()
+    assert "ocl_run" in names and "ocl_open" in names
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_settings_layout_matches_reference_struct(tcoh, cref):
+    """struct settings of local_proto.h:33-49 in both bindings."""
+    a, b = tcoh.api.Settings, cref.OSettings
+    assert C.sizeof(a) == C.sizeof(b)
+    for (fa, _), (fb, _) in zip(a._fields_, b._fields_):
+        assert getattr(a, fa).offset == getattr(b, fb).offset
+
+
+@pytest.mark.parametrize("n", [3, 6, 10, 30, 45, 64])
+@pytest.mark.parametrize("test", ["ks", "ad", "tlog"])
+def test_shp_constants_match_oracle(tcoh, cref, n, test):
+    for alpha in (0.01, 0.05, 0.2):
+        s = tcoh.make_settings(n, window=(21, 7), shp_test=test, alpha=alpha, min_shp=3)
+        o = cref.settings(n, shp_test=test, alpha=alpha, min_shp=3)
+        for f in ("ks_max", "ad_norm", "ad_sigma", "ad_crit", "tlog_crit"):
+            assert getattr(s, f) == getattr(o, f), f
+
+
+def test_settings_derivation(tcoh):
+    """main.c:585-608."""
+    s = tcoh.make_settings(8, window=(5, 3), min_shp=4, pairs="all")
+    assert s.min_looks == 8 and s.npairs == 28 and s.pairs_all == 1
+    s = tcoh.make_settings(8, min_shp=30, pairs="consecutive")
+    assert s.min_looks == 30 and s.npairs == 7
+    with pytest.raises(ValueError):
+        tcoh.make_settings(8, window=(4, 3))
+    with pytest.raises(ValueError):
+        tcoh.make_settings(40, window=(5, 5))
+    assert tcoh.pair_list(4, "all")[:3] == [(0, 1), (0, 2), (0, 3)]
+
+
+def test_prepare_strip_matches_oracle(tcoh, cref):
+    """read_strip (main.c:52-127): validity, sorted sqrtf amplitudes and the
+    double-precision rotation, bit for bit."""
+    stack = tcoh.synth.ds_stack(6, 0.6, rows=20, cols=17, seed=11)
+    stack[1, 3:5, 2:6] = np.nan
+    stack[4, 7, 7] = 0
+    rng = np.random.default_rng(0)
+    phase = rng.uniform(-10, 10, stack.shape)
+    elev_null = np.zeros(stack.shape[1:], np.uint8)
+    elev_null[12, 3] = 1
+    i = np.ascontiguousarray(stack.real, np.float32)
+    q = np.ascontiguousarray(stack.imag, np.float32)
+    for row0, prow, halo, ph, en in ((-3, 26, 2, None, None), (5, 9, 3, phase, elev_null)):
+        got = tcoh.prepare_strip(i, q, row0, prow, halo, ph, en)
+        want = cref.prepare_strip(i, q, row0, prow, halo, ph, en)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+
+
+def test_orbit_state_and_geometry_phases_match_oracle(tcoh, cref):
+    api = tcoh.api
+    n = 13
+    t = np.arange(n) * 10.0 + 1000.0
+    pos = np.stack([7e6 * np.cos(t / 900.0), 1e5 + 0 * t, 7e6 * np.sin(t / 900.0)], 1)
+    vel = np.gradient(pos, axis=0) / 10.0
+    o = api.Orbit()
+    o.n = n
+    tt = np.ascontiguousarray(t)
+    pp = np.ascontiguousarray(pos)
+    vv = np.ascontiguousarray(vel)
+    o.t = tt.ctypes.data_as(C.POINTER(C.c_double))
+    o.pos = pp.ctypes.data_as(C.POINTER(C.c_double))
+    o.vel = vv.ctypes.data_as(C.POINTER(C.c_double))
+    for tq in (1000.0, 1033.3, 1061.0, 1120.0):
+        p1, v1 = np.zeros(3), np.zeros(3)
+        p2, v2 = np.zeros(3), np.zeros(3)
+        api.lib().orbit_state(C.byref(o), tq, p1.ctypes.data, v1.ctypes.data)
+        cref.lib().oracle_orbit_state(n, tt, pp.reshape(-1), vv.reshape(-1), tq, p2, v2)
+        assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+    # geometry_phases on a hand-made grid
+    g = api.Geometry()
+    g.ndates, g.grid_rows, g.grid_cols, g.step, g.nheights = 3, 4, 5, 16, 3
+    for k, h in enumerate((0.0, 250.0, 500.0)):
+        g.heights[k] = h
+    grid = np.ascontiguousarray(rng_grid := np.random.default_rng(1).normal(
+        size=4 * 5 * 3 * 3))
+    g.phase = grid.ctypes.data_as(C.POINTER(C.c_double))
+    hs = np.array([0.0, 250.0, 500.0])
+    for row, col, h in ((0, 0, 0.0), (17, 40, 123.0), (47, 63, 480.0)):
+        a, b = np.zeros(3), np.zeros(3)
+        api.lib().geometry_phases(C.byref(g), row, col, h, a.ctypes.data)
+        cref.lib().oracle_geometry_phases(3, 5, 3, hs, rng_grid, row, col, h, b)
+        assert np.array_equal(a, b)

@@ -1,0 +1,285 @@
+"""Pins of the CPU oracle (no GPU): known answers, scipy, closed forms,
+brute force, and the FP32 restatement against the FP64 oracle."""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import strip_of
+
+TOLERANCE = 1e-4  # largest |d gamma|, |d pair| vs FP64 (reference test:27)
+
+# Appendix values computed independently from stats.c's closed forms.
+KAT = {
+    8: (5, 1.562500000e-02, 6.812341293e-01),
+    10: (6, 1.000000000e-02, 6.962061929e-01),
+    12: (6, 6.944444444e-03, 7.063865824e-01),
+    20: (8, 2.500000000e-03, 7.273664994e-01),
+    30: (10, 1.111111111e-03, 7.382626231e-01),
+    45: (12, 4.938271605e-04, 7.457303009e-01),
+    60: (14, 2.777777778e-04, 7.495401734e-01),
+}
+
+
+@pytest.mark.parametrize("n", sorted(KAT))
+def test_shp_constants_kat(cref, n):
+    s = cref.settings(n, shp_test="ad")
+    ks, norm, sigma = KAT[n]
+    assert s.ks_max == ks
+    assert "%.9e" % s.ad_norm == "%.9e" % norm
+    assert "%.9e" % s.ad_sigma == "%.9e" % sigma
+    assert s.ad_crit == pytest.approx(1.961, abs=1e-12)
+    assert "%.9e" % s.tlog_crit == "1.959963985e+00"
+
+
+def test_constants_match_scipy(cref):
+    stats = pytest.importorskip("scipy.stats")
+    L = cref.lib()
+    for alpha in (0.001, 0.01, 0.05, 0.1, 0.25):
+        assert L.oracle_normal_quantile(alpha) == pytest.approx(
+            stats.norm.isf(alpha / 2), rel=1e-12)
+    # Table points of Scholz & Stephens as scipy interpolates them.
+    assert L.oracle_ad_critical(0.05) == pytest.approx(1.961)
+    assert L.oracle_ad_critical(0.01) == pytest.approx(3.752)
+    assert math.isnan(L.oracle_ad_critical(0.3))
+    for n in (4, 12, 30):
+        crit = math.sqrt(-0.5 * math.log(0.025)) * math.sqrt(2.0 / n)
+        assert cref.settings(n).ks_max == math.floor(crit * n)
+
+
+@pytest.mark.filterwarnings("ignore:p-value capped")
+@pytest.mark.filterwarnings("ignore:p-value floored")
+def test_shp_statistics_match_scipy(cref):
+    """reference test:30-46, on the FP32 restatement."""
+    stats = pytest.importorskip("scipy.stats")
+    L = cref.lib()
+    rng = np.random.default_rng(1)
+    n = 12
+    s = cref.settings(n, shp_test="ad")
+    norm, sig = L.oracle_kernel_float(s.ad_norm), L.oracle_kernel_float(s.ad_sigma)
+    for _ in range(200):
+        a = np.sort(rng.rayleigh(size=n)).astype(np.float32)
+        b = np.sort(rng.rayleigh(scale=1.5, size=n)).astype(np.float32)
+        expected = stats.anderson_ksamp([a.astype(np.float64), b.astype(np.float64)],
+                                        midrank=False).statistic
+        got = L.oracle_ad_statistic(n, norm, sig, a, b)
+        assert got == pytest.approx(expected, rel=2e-5, abs=2e-5)
+        d = stats.ks_2samp(a, b).statistic
+        assert L.oracle_ks_accept(n, s.ks_max, a, b) == int(round(d * n) <= s.ks_max)
+
+
+def ks_closed_form(a, b, k):
+    """The GPU kernel's formulation: for i >= K, b[i-K] < a[i], a[i-K] <= b[i]."""
+    if k < 1:
+        return 0
+    for i in range(k, len(a)):
+        if not (b[i - k] < a[i] and a[i - k] <= b[i]):
+            return 0
+    return 1
+
+
+def test_ks_closed_form_equals_merge_walk(cref):
+    L = cref.lib()
+    rng = np.random.default_rng(7)
+    for trial in range(4000):
+        n = int(rng.integers(3, 65))
+        k = int(rng.integers(0, n + 1))
+        if trial % 3 == 0:  # heavy ties
+            a = np.sort(rng.integers(0, 4, n)).astype(np.float32)
+            b = np.sort(rng.integers(0, 4, n)).astype(np.float32)
+        else:
+            a = np.sort(rng.rayleigh(size=n)).astype(np.float32)
+            b = np.sort(rng.rayleigh(scale=rng.uniform(0.6, 1.6), size=n)).astype(np.float32)
+        assert L.oracle_ks_accept(n, k, a, b) == ks_closed_form(a, b, k)
+    # exhaustive: every pair of sorted 0/1/2 samples of length 4
+    vals = [np.array(v, np.float32) for v in np.ndindex(3, 3, 3, 3)]
+    vals = [np.sort(v) for v in vals]
+    for a in vals:
+        for b in vals:
+            for k in range(0, 5):
+                assert L.oracle_ks_accept(4, k, a, b) == ks_closed_form(a, b, k)
+
+
+def test_tlog_matches_double_welch(cref):
+    """Welch t on the sorted log amplitudes (tcoh.cl:77-95) vs float64."""
+    L = cref.lib()
+    rng = np.random.default_rng(3)
+    crit = cref.settings(10, alpha=0.01).tlog_crit
+    cf = L.oracle_kernel_float(crit)
+    agree = 0
+    for _ in range(500):
+        a = np.sort(rng.rayleigh(size=10)).astype(np.float32)
+        b = np.sort(rng.rayleigh(scale=rng.uniform(0.7, 1.5), size=10)).astype(np.float32)
+        la, lb = np.log(a.astype(np.float64)), np.log(b.astype(np.float64))
+        se = math.sqrt((la.var(ddof=1) + lb.var(ddof=1)) / 10)
+        t = abs(la.mean() - lb.mean()) / se
+        if abs(t - crit) < 1e-4:
+            continue
+        assert L.oracle_tlog_accept(10, cf, a, b) == int(t <= crit)
+        agree += 1
+    assert agree > 450
+
+
+def test_logf_is_rounded_double_log(cref):
+    L = cref.lib()
+    for x in (1.0, 2.5, 1e-20, 3.0e30, 7.0):
+        assert L.oracle_logf(x) == np.float32(math.log(np.float32(x)))
+
+
+# ---- FP64 oracle pins -------------------------------------------------------
+
+def test_fp64_gamma_is_one_for_consistent_phases(fp64):
+    """T = Gamma o u u^H, Gamma real positive definite, |u_i| = 1: the EVD
+    eigenvector is D_u times Gamma's (positive, Perron) top eigenvector, and
+    by Fiedler's inequality Gamma^-1 o Gamma >= I with eigenvector 1, so
+    EMI returns u; both give gamma = 1 and the phases of u."""
+    rng = np.random.default_rng(0)
+    n = 7
+    t_days = 12.0 * np.arange(n)
+    gam = 0.6 * np.exp(-np.abs(t_days[:, None] - t_days[None, :]) / 48.0) + 0.3
+    np.fill_diagonal(gam, 1.0)
+    u = np.exp(1j * rng.uniform(-np.pi, np.pi, (5, n)))
+    t = gam[None] * u[:, :, None] * u[:, None, :].conj()
+    want = u * np.conj(u[:, n // 2])[:, None]
+    for emi in (False, True):
+        z = fp64.estimate(t, emi=emi)
+        assert np.allclose(fp64.gamma(t, z), 1.0, atol=1e-9)
+        assert np.allclose(z, want, atol=1e-9)
+
+
+def test_fp64_phasor_conventions(fp64):
+    # datum = date N/2; a negligible datum falls back to the LAST largest
+    u = np.array([[1 + 0j, 1j, 2.0, 1e-6, -1j]])
+    z = fp64.phasors(u)
+    assert np.allclose(z[0, 2], 1.0)
+    assert np.allclose(z[0, 1], 1j)
+    u = np.array([[2 + 0j, 1j, 1e-9, 2j, 0.5]])
+    z = fp64.phasors(u)           # |u_2| tiny -> datum = last of |u| == 2 (index 3)
+    assert np.allclose(z[0, 3], 1.0)
+    assert np.allclose(z[0, 0], -1j)
+    assert z[0, 2] == 1.0         # negligible component -> phasor 1
+
+
+def test_fp64_gamma_global_phase_invariance(fp64):
+    rng = np.random.default_rng(2)
+    n, p = 6, 50
+    x = rng.standard_normal((p, 40, n)) + 1j * rng.standard_normal((p, 40, n))
+    c = np.einsum("pki,pkj->pij", x, x.conj())
+    t = fp64.coherence(c, np.full(p, 40), bias=False)
+    z = fp64.estimate(t, emi=False)
+    g1 = fp64.gamma(t, z)
+    g2 = fp64.gamma(t, fp64.phasors(z * np.exp(0.7j)))
+    assert np.allclose(g1, g2, atol=1e-12)
+    # EVD eigenvector against power iteration on T + 2I
+    v = np.ones((p, n), complex)
+    for _ in range(3000):
+        v = np.einsum("pij,pj->pi", t + 2 * np.eye(n), v)
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+    assert np.allclose(fp64.phasors(v), z, atol=1e-6)
+
+
+def test_fp64_coherence_and_bias_closed_form(fp64):
+    c = np.array([[[4.0, 2 + 2j], [2 - 2j, 9.0]]])
+    t = fp64.coherence(c, np.array([10]), bias=False)
+    assert np.allclose(t[0, 0, 1], (2 + 2j) / 6.0)
+    tb = fp64.coherence(c, np.array([10]), bias=True)
+    m2 = abs((2 + 2j) / 6.0) ** 2
+    m2c = (10 * m2 - 1) / 9
+    assert np.allclose(tb[0, 0, 1], (2 + 2j) / 6.0 * math.sqrt(m2c / m2))
+    # small coherence with few looks: zeroed by the correction
+    c2 = np.array([[[1.0, 0.1], [0.1, 1.0]]])
+    assert fp64.coherence(c2, np.array([5]), bias=True)[0, 0, 1] == 0
+
+
+def test_fp64_connected(fp64):
+    t = np.eye(4, dtype=complex)[None].repeat(2, 0)
+    t[0, 0, 1] = t[0, 1, 0] = 0.3
+    t[0, 1, 2] = t[0, 2, 1] = 0.2j
+    t[0, 3, 2] = t[0, 2, 3] = 0.1
+    t[1, 0, 1] = t[1, 1, 0] = 0.3
+    t[1, 2, 3] = t[1, 3, 2] = 0.5
+    assert list(fp64.connected(t)) == [True, False]
+
+
+def test_fp64_emi_inverse_branches(fp64):
+    g = np.array([[[1.0, 0.5], [0.5, 1.0]], [[1.0, 1.0], [1.0, 1.0]]]) + 0j
+    inv, ok = fp64.abs_inverse(g)
+    assert list(ok) == [True, False]
+    assert np.allclose(inv[0], np.linalg.inv(g[0].real))
+    # singular: pseudo-inverse of [[1,1],[1,1]] = [[.25,.25],[.25,.25]]
+    assert np.allclose(inv[1], 0.25)
+
+
+# ---- FP32 restatement of the kernel against the FP64 oracle -----------------
+
+CASES = [
+    dict(n=6, g=0.7, seed=0, opts={}),
+    dict(n=6, g=0.7, seed=0, opts=dict(shp_test="ad", estimator="emi", min_shp=9, window=(9, 5))),
+    dict(n=6, g=0.7, seed=0, opts=dict(shp_test="tlog", alpha=0.01, min_shp=9, window=(7, 5))),
+    dict(n=6, g=0.3, seed=3, opts=dict(bias=True, estimator="evd", window=(5, 3), min_shp=6)),
+    dict(n=6, g=0.3, seed=3, opts=dict(bias=True, estimator="emi", window=(5, 3), min_shp=6)),
+    dict(n=5, g=0.7, seed=20, opts=dict(min_shp=5, pairs="all")),
+    dict(n=10, g=0.95, seed=5, opts={}),
+]
+
+
+def run_oracles(cref, fp64, tcoh_synth, case):
+    n, opts = case["n"], dict(case["opts"])
+    stack = tcoh_synth.ds_stack(n, case["g"], seed=case["seed"])
+    window = opts.get("window", (21, 7))
+    s = cref.settings(n, window=window, shp_test=opts.get("shp_test", "ks"),
+                      alpha=opts.get("alpha", 0.05), min_shp=opts.get("min_shp", 20),
+                      estimator=opts.get("estimator", "evd"), bias=opts.get("bias", False),
+                      pairs=opts.get("pairs", "consecutive"))
+    slc, amp, valid, prow, pcols = strip_of(stack, window)
+    rows, cols = stack.shape[1:]
+    coh, count, pairs, mask, flips = cref.run(s, pcols, slc, amp, valid, prow, rows, cols)
+    c64, p64 = fp64.run(slc, count, mask, cref.settings_dict(s), pcols, rows, cols)
+    return s, coh, count, pairs, c64, p64, flips
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
+def test_fp32_restatement_within_tolerance_of_fp64(cref, fp64, tcoh, case):
+    s, coh, count, pairs, c64, p64, flips = run_oracles(cref, fp64, tcoh.synth, case)
+    assert flips == 0
+    assert np.array_equal(np.isnan(coh), np.isnan(c64))
+    assert np.nanmax(np.abs(coh - c64)) <= TOLERANCE
+    assert np.array_equal(np.isnan(pairs.T), np.isnan(p64))
+    assert np.nanmax(np.abs(pairs.T - p64)) <= TOLERANCE
+    assert count.max() <= s.win_az * s.win_rg
+
+
+def test_bias_unlinked_dates_give_nan_with_finite_pairs(cref, fp64, tcoh):
+    """reference test:357-378: with -b, unlinked dates -> NaN gamma, pairs
+    still defined."""
+    s, coh, count, pairs, c64, p64, _ = run_oracles(cref, fp64, tcoh.synth, CASES[3])
+    unlinked = np.isnan(coh) & (count >= s.min_looks)
+    assert unlinked.any()
+    assert np.isfinite(pairs[0][unlinked]).all()
+
+
+def test_coherent_and_incoherent(cref, fp64, tcoh):
+    """reference test:138-151."""
+    means = {}
+    for g in (0.95, 0.0):
+        stack = tcoh.synth.ds_stack(10, g, seed=5)
+        s = cref.settings(10)
+        slc, amp, valid, prow, pcols = strip_of(stack, (21, 7))
+        coh, *_ = cref.run(s, pcols, slc, amp, valid, prow, 30, 24)
+        means[g] = np.nanmean(coh)
+    assert means[0.95] > 0.9 and means[0.0] < 0.5
+
+
+def test_nulls_and_zero_samples(cref, tcoh):
+    """reference test:154-169."""
+    stack = tcoh.synth.ds_stack(5, 0.8, seed=6)
+    stack[2, 10:13, 4:9] = np.nan
+    stack[4, 20, 20] = 0
+    s = cref.settings(5, min_shp=5)
+    slc, amp, valid, prow, pcols = strip_of(stack, (21, 7))
+    coh, count, *_ = cref.run(s, pcols, slc, amp, valid, prow, 30, 24)
+    coh, count = coh.reshape(30, 24), count.reshape(30, 24)
+    assert np.isnan(coh[10:13, 4:9]).all() and (count[10:13, 4:9] == -1).all()
+    assert np.isnan(coh[20, 20]) and count[20, 20] == -1
+    assert np.isfinite(coh[25, 5])
