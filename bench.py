@@ -173,7 +173,7 @@ def main():
                                  args.warmup, torch)
         forms[form] = (ms, st_ms, nl, steps)
         maps[form] = (coh.cpu().numpy(), count.cpu().numpy(), pairs.cpu().numpy())
-        log("%s: %.2f ms/step, stages prep/shp/cov/eig ms %s" %
+        log("%s: %.2f ms/step, stages prep/shp/link ms %s" %
             (form, ms, np.round(st_ms, 3).tolist()))
 
     ms, st_ms, nl, steps = forms[args.precision]
@@ -183,24 +183,26 @@ def main():
     est = int(np.sum(count_h >= s.min_looks))
     looks = count_h[count_h >= s.min_looks].astype(np.float64)
 
-    # Dominant kernel of the timed form and its algorithmic FLOP rate.
-    stage_names = ["prep", "shp", "cov", "eig"]
-    dom = int(np.argmax(st_ms))
-    if dom == 3:
-        flop = est * flops_eigen(n)
-        peak, kname = PEAK_FP32, "k_eig (FP32 vector ALU)"
-    elif dom == 2:
-        flop = float(np.sum(4.0 * n * (n + 1) * looks))
-        if args.precision == "plain":
-            peak, kname = PEAK_FP32, "k_cov_plain (FP32 vector ALU)"
-        else:
-            # three hi/lo products on the matrix cores
-            flop *= 3.0
-            peak, kname = PEAK_F16_MFMA, "k_cov_mfma (split-16-bit MFMA, 3 products)"
+    # The dominant kernel is the fused covariance + linking kernel k_link.
+    # Algorithmic FLOPs per estimated pixel (L = its SHP count):
+    # covariance 4N(N+1)L, linking (16/3)N^3 + 8N(N-1). For the mixed form
+    # the bound is the matrix-core time of the three split products of the
+    # covariance plus the FP32 vector time of the linking; "peak" is the
+    # FLOP rate at that bound, so frac = bound time / measured time.
+    stage_names = ["prep", "shp", "link", "unused"]
+    dom = 2
+    flop_cov = float(np.sum(4.0 * n * (n + 1) * looks))
+    flop_link = est * flops_eigen(n)
+    if args.precision == "plain":
+        t_bound = (flop_cov + flop_link) / PEAK_FP32
+        kname = "k_link plain (FP32 vector ALU)"
     else:
-        flop, peak, kname = 0.0, PEAK_FP32, "k_" + stage_names[dom]
+        t_bound = 3.0 * flop_cov / PEAK_F16_MFMA + flop_link / PEAK_FP32
+        kname = "k_link %s (split-16-bit MFMA covariance + FP32 linking)" % args.precision
+    flop = flop_cov + flop_link
     avg_launch_s = st_ms[dom] * 1e-3 / max(1.0, nl[dom])
-    achieved = flop / max(1.0, nl[dom]) / avg_launch_s if flop else 0.0
+    achieved = flop / max(1.0, nl[dom]) / avg_launch_s
+    peak = flop / t_bound
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                 "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,

@@ -25,7 +25,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtcoh_hip.so")
+# TCOH_LIB selects another build of the library (e.g. a debug build).
+LIB_PATH = os.environ.get("TCOH_LIB") or os.path.join(HERE, "libtcoh_hip.so")
 
 SHP_TESTS = {"ks": 0, "ad": 1, "tlog": 2}
 PRECISIONS = {"plain": 0, "f16": 1, "bf16": 2}

@@ -7,12 +7,11 @@
  *   prep      amplitudes to date-planar order (+ log statistics for tlog)
  *   shp       SHP mask and count per output pixel
  *   [split]   hi/lo 16-bit copy of the samples (mixed precision only)
- *   per batch of output pixels (bounded scratch):
- *     cov     coherence matrix T and pair coherences
- *     eig     phase linking and temporal coherence
+ *   link      per band of rows, fused: coherence matrix T in LDS, pair
+ *             coherences, phase linking and temporal coherence
  * Buffers only grow (ocl.c:264-277). The reference splits its launches
- * into ~0.5 s chunks against GPU watchdogs (ocl.c:329-363); here the
- * batches bound every launch instead.
+ * into ~0.5 s chunks against GPU watchdogs (ocl.c:329-363); here bands of
+ * about a million pixels bound every launch instead.
  */
 
 #include <math.h>
@@ -23,6 +22,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -123,8 +123,6 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
-/* Largest T scratch of one batch. */
-const size_t TBUF_BYTES = (size_t)512 << 20;
 
 } // namespace
 
@@ -172,7 +170,7 @@ struct ocl {
     /* ocl_run's transfer buffers */
     DevBuf slc, amp, valid, coh, count, pairs;
     /* scratch */
-    DevBuf ampT, lstat, mask, adtab, tbuf, hl, absmax;
+    DevBuf ampT, lstat, mask, adtab, hl, absmax;
     int adtab_n;
 };
 
@@ -190,9 +188,11 @@ static int env_precision()
 }
 
 static bool eig_supported(int n);
-static hipError_t launch_eig(const KParams &kp, const float2 *tbuf,
-                             const int *count, float *coh, int pix0, int npix,
-                             hipStream_t s);
+static hipError_t launch_link(const KParams &kp, const float2 *slc,
+                              const uint2 *hl, const uint32_t *mask,
+                              const int *count, float *coh, float *pairs,
+                              int row0, int nrows, int precision,
+                              hipStream_t s);
 
 extern "C" void ocl_list_devices(void)
 {
@@ -272,8 +272,17 @@ extern "C" struct ocl *ocl_open(int platform, int device,
             /* The strip's samples (8N bytes a pixel) take about 40 % of
                what one strip needs on the device (amplitudes twice,
                masks, outputs, T scratch). */
-            if (max_alloc)
+            /* ... and the SHP kernel indexes the strip's date planes with
+               32-bit offsets: samples of at most INT32_MAX / bucket
+               pixels (main.c sizes strips by max_alloc / (8N) pixels). */
+            if (max_alloc) {
+                const size_t cap = (size_t)(INT32_MAX / shp_bucket(s->ndates)) *
+                                   s->ndates * 8;
+
                 *max_alloc = (size_t)((double)fr * 0.4);
+                if (*max_alloc > cap)
+                    *max_alloc = cap;
+            }
             return o;
         },
         nullptr);
@@ -368,9 +377,13 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
     kp.tlog_crit = kernel_float(s.tlog_crit);
 
     const size_t npad = (size_t)kp.npad, nout = (size_t)rows * cols;
-    const int n = kp.n, ntri = n * (n - 1) / 2;
+    const int n = kp.n;
 
-    o->ampT.ensure(npad * n * sizeof(float), "hipMalloc(ampT)");
+    if (npad * shp_bucket(n) > (size_t)INT32_MAX)
+        fail("Strip of %zu pixels too large for %d dates (at most %d pixels; "
+             "use strips within ocl_open's max_alloc)",
+             npad, n, INT32_MAX / shp_bucket(n));
+    o->ampT.ensure(npad * shp_bucket(n) * sizeof(float), "hipMalloc(ampT)");
     o->mask.ensure(nout * kp.words * sizeof(uint32_t), "hipMalloc(mask)");
     if (kp.test == 2)
         o->lstat.ensure(npad * sizeof(float2), "hipMalloc(lstat)");
@@ -385,12 +398,6 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
         free(q);
         o->adtab_n = n;
     }
-    size_t batch = TBUF_BYTES / ((size_t)ntri * sizeof(float2));
-    if (batch > nout)
-        batch = nout;
-    batch = (batch + 63) / 64 * 64;
-    o->tbuf.ensure(batch * ntri * sizeof(float2), "hipMalloc(tbuf)");
-
     StageTimer &tm = o->timer;
     const bool timing = o->timing != 0;
     auto mark = [&](int stage, bool start) {
@@ -452,29 +459,21 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
         mark(0, false);
     }
 
-    for (size_t p0 = 0; p0 < nout; p0 += batch) {
-        const int np = (int)(p0 + batch <= nout ? batch : nout - p0);
+    /* Fused covariance + linking, in launches of about a million output
+       pixels (each well under the GPU watchdog, ocl.c:22-23). */
+    const int chunk = std::max(1, (1 << 20) / cols);
+
+    for (int r0 = 0; r0 < rows; r0 += chunk) {
+        const int nr = std::min(chunk, rows - r0);
 
         mark(2, true);
-        if (mixed)
-            check(launch_cov_mfma(kp, o->hl.as<uint32_t>(),
-                                  o->mask.as<uint32_t>(), (const int *)count,
-                                  o->tbuf.as<float2>(), (float *)pairs,
-                                  (int)p0, np,
-                                  o->precision == TCOH_MIXED_BF16, st),
-                  "covariance kernel");
-        else
-            check(launch_cov_plain(kp, (const float2 *)slc,
-                                   o->mask.as<uint32_t>(), (const int *)count,
-                                   o->tbuf.as<float2>(), (float *)pairs,
-                                   (int)p0, np, st),
-                  "covariance kernel");
+        check(launch_link(kp, (const float2 *)slc,
+                          (const uint2 *)o->hl.as<uint32_t>(),
+                          o->mask.as<uint32_t>(), (const int *)count,
+                          (float *)coh, (float *)pairs, r0, nr, o->precision,
+                          st),
+              "link kernel");
         mark(2, false);
-        mark(3, true);
-        check(launch_eig(kp, o->tbuf.as<float2>(), (const int *)count,
-                         (float *)coh, (int)p0, np, st),
-              "eigen kernel");
-        mark(3, false);
     }
     if (timing) {
         for (int i = 0; i < 4; i++) {
@@ -557,7 +556,7 @@ extern "C" void ocl_close(struct ocl *o)
         (void)hipStreamSynchronize(o->stream);
         DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
                           &o->count, &o->pairs, &o->ampT,  &o->lstat,
-                          &o->mask, &o->adtab, &o->tbuf,  &o->hl,
+                          &o->mask, &o->adtab, &o->hl,
                           &o->absmax};
         for (DevBuf *b : bufs)
             b->release();
@@ -571,11 +570,13 @@ extern "C" void ocl_close(struct ocl *o)
    to X(3) X(4) ... for the depths the build compiled. */
 #define TCOH_EIG_CASE(NN)                                                    \
     case NN:                                                                 \
-        return launch_eig_n<NN>(kp, tbuf, count, coh, pix0, npix, s);
+        return launch_link_n<NN>(kp, slc, hl, mask, count, coh, pairs, row0, \
+                                 nrows, precision, s);
 
-static hipError_t launch_eig(const KParams &kp, const float2 *tbuf,
-                             const int *count, float *coh, int pix0, int npix,
-                             hipStream_t s)
+static hipError_t launch_link(const KParams &kp, const float2 *slc,
+                              const uint2 *hl, const uint32_t *mask,
+                              const int *count, float *coh, float *pairs,
+                              int row0, int nrows, int precision, hipStream_t s)
 {
     switch (kp.n) {
         TCOH_EIG_FOR_EACH(TCOH_EIG_CASE)

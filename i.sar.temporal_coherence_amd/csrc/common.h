@@ -40,6 +40,23 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* The same value, but held in a vector register: offsets derived from it
+   are computed per lane instead of being hoisted into scalar registers
+   (which a fully unrolled loop over dates would otherwise spill). The
+   empty asm only fixes the register class; it emits no instruction. */
+__device__ __forceinline__ int in_vgpr(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+/* Date planes of the planar amplitude copy: the SHP kernel's depth
+   bucket, so that loads of planes n..NB-1 stay in bounds. */
+__host__ __device__ __forceinline__ int shp_bucket(int n)
+{
+    return n <= 16 ? 16 : (n <= 32 ? 32 : 64);
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -52,9 +69,41 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
 }
 
 /* Reductions over the G consecutive lanes of a group (G a power of two
-   dividing 64). */
+   dividing 64). For G <= 16 the group is inside one 16-lane DPP row and
+   the all-reduce is four DPP moves: quad_perm [1,0,3,2] (xor 1),
+   quad_perm [2,3,0,1] (xor 2), row_half_mirror (lane i <-> 7-i of each
+   8) and row_mirror (lane i <-> 15-i); every lane ends with the total.
+   All lanes of the wavefront must be active. */
+#define DPP_XOR1 0xB1
+#define DPP_XOR2 0x4E
+#define DPP_HALF_MIRROR 0x141
+#define DPP_MIRROR 0x140
+
+template <int CTRL> __device__ __forceinline__ float dpp_f(float v)
+{
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                           0xF, 0xF, false));
+}
+
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
 template <int G> __device__ __forceinline__ float gsum(float v)
 {
+    if (G <= 16) {
+        if (G >= 2)
+            v += dpp_f<DPP_XOR1>(v);
+        if (G >= 4)
+            v += dpp_f<DPP_XOR2>(v);
+        if (G >= 8)
+            v += dpp_f<DPP_HALF_MIRROR>(v);
+        if (G >= 16)
+            v += dpp_f<DPP_MIRROR>(v);
+        return v;
+    }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1)
         v += __shfl_xor(v, o, G);
@@ -71,6 +120,17 @@ template <int G> __device__ __forceinline__ double gsumd(double v)
 
 template <int G> __device__ __forceinline__ float gmax(float v)
 {
+    if (G <= 16) {
+        if (G >= 2)
+            v = fmaxf(v, dpp_f<DPP_XOR1>(v));
+        if (G >= 4)
+            v = fmaxf(v, dpp_f<DPP_XOR2>(v));
+        if (G >= 8)
+            v = fmaxf(v, dpp_f<DPP_HALF_MIRROR>(v));
+        if (G >= 16)
+            v = fmaxf(v, dpp_f<DPP_MIRROR>(v));
+        return v;
+    }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1)
         v = fmaxf(v, __shfl_xor(v, o, G));
@@ -79,6 +139,17 @@ template <int G> __device__ __forceinline__ float gmax(float v)
 
 template <int G> __device__ __forceinline__ float gmin(float v)
 {
+    if (G <= 16) {
+        if (G >= 2)
+            v = fminf(v, dpp_f<DPP_XOR1>(v));
+        if (G >= 4)
+            v = fminf(v, dpp_f<DPP_XOR2>(v));
+        if (G >= 8)
+            v = fminf(v, dpp_f<DPP_HALF_MIRROR>(v));
+        if (G >= 16)
+            v = fminf(v, dpp_f<DPP_MIRROR>(v));
+        return v;
+    }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1)
         v = fminf(v, __shfl_xor(v, o, G));
@@ -87,6 +158,17 @@ template <int G> __device__ __forceinline__ float gmin(float v)
 
 template <int G> __device__ __forceinline__ int gmaxi(int v)
 {
+    if (G <= 16) {
+        if (G >= 2)
+            v = max(v, dpp_i<DPP_XOR1>(v));
+        if (G >= 4)
+            v = max(v, dpp_i<DPP_XOR2>(v));
+        if (G >= 8)
+            v = max(v, dpp_i<DPP_HALF_MIRROR>(v));
+        if (G >= 16)
+            v = max(v, dpp_i<DPP_MIRROR>(v));
+        return v;
+    }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1)
         v = max(v, __shfl_xor(v, o, G));
@@ -102,6 +184,51 @@ gor64(unsigned long long v)
     return v;
 }
 
+/* XCD-aware tiling of a band of R rows x nbx tiles (a tile = TW
+   consecutive pixels of one row). The hardware deals work-groups to the 8
+   XCDs round-robin (id % 8); with a grid of xcd_grid() groups, group id
+   maps to logical tile x*per + id/8 of XCD x, and logical tiles run
+   stripe by stripe (stripes nbx/8 tiles wide, top to bottom), so each
+   XCD's L2 holds the windows of one vertical stripe. Returns false for
+   the padding groups. */
+#define NXCD 8
+
+__host__ __device__ __forceinline__ int xcd_stripe(int nbx)
+{
+    return (nbx + NXCD - 1) / NXCD;
+}
+
+__host__ __device__ __forceinline__ int xcd_logical(int R, int nbx)
+{
+    const int sb = xcd_stripe(nbx);
+
+    return ((nbx + sb - 1) / sb) * R * sb;
+}
+
+__host__ __forceinline__ int xcd_grid(int R, int nbx)
+{
+    const int per = (xcd_logical(R, nbx) + NXCD - 1) / NXCD;
+
+    return per * NXCD;
+}
+
+__device__ __forceinline__ bool xcd_tile(int bid, int R, int nbx, int *row,
+                                         int *tx)
+{
+    const int nl = xcd_logical(R, nbx);
+    const int per = (nl + NXCD - 1) / NXCD;
+    const int l = (bid % NXCD) * per + bid / NXCD;
+    const int sb = xcd_stripe(nbx);
+
+    if (l >= nl)
+        return false;
+    const int s = l / (R * sb), rem = l - s * (R * sb);
+
+    *row = rem / sb;
+    *tx = s * sb + rem % sb;
+    return *tx < nbx;
+}
+
 /* Offset of T_ij (i < j) in the packed strict upper triangle, row by row:
    (0,1),(0,2)...(0,N-1),(1,2)... -- also the pair order of pairs_mode=all
    (tcoh.cl:19-23). */
@@ -111,11 +238,7 @@ __host__ __device__ __forceinline__ int upper_index(int n, int i, int j)
 }
 
 /* Kernel launchers (defined in the .hip files). Each returns a HIP error
-   code; stream is the handle's stream. */
-struct LaunchCtx {
-    hipStream_t stream;
-};
-
+   code; s is the handle's stream. */
 hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
                        const unsigned char *valid, float2 *lstat,
                        hipStream_t s);
@@ -123,21 +246,17 @@ hipError_t launch_shp(const KParams &kp, const float *ampT,
                       const unsigned char *valid, const float2 *lstat,
                       const float *adtab, uint32_t *mask, int *count,
                       hipStream_t s);
-hipError_t launch_cov_plain(const KParams &kp, const float2 *slc,
-                            const uint32_t *mask, const int *count,
-                            float2 *tbuf, float *pairs, int pix0, int npix,
-                            hipStream_t s);
 hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
                         int bf16, float scale, hipStream_t s);
-hipError_t launch_cov_mfma(const KParams &kp, const uint32_t *hl,
-                           const uint32_t *mask, const int *count,
-                           float2 *tbuf, float *pairs, int pix0, int npix,
-                           int bf16, hipStream_t s);
-template <int N>
-hipError_t launch_eig_n(const KParams &kp, const float2 *tbuf,
-                        const int *count, float *coh, int pix0, int npix,
-                        hipStream_t s);
 hipError_t launch_absmax(const float2 *slc, size_t n, unsigned int *out,
+                         hipStream_t s);
+/* Fused covariance + linking of output rows row0 .. row0+nrows-1
+   (precision: 0 plain, 1 split FP16, 2 split BF16), one instantiation
+   per stack depth N. */
+template <int N>
+hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
+                         const uint32_t *mask, const int *count, float *coh,
+                         float *pairs, int row0, int nrows, int precision,
                          hipStream_t s);
 
 #endif

@@ -1,10 +1,12 @@
-/* Phase linking and temporal coherence on gfx950 (tcoh.cl:641-744).
+/* Phase linking and temporal coherence on gfx950 (tcoh.cl:641-744):
+ * device functions of the linking stage of k_link.
  *
  * G lanes per pixel; lane l of a group owns the columns c = l + m*G
  * (m < CPL) of the pixel's N x N Hermitian matrix and keeps them, both
- * triangles, in registers. Row-wise data every lane needs (Householder
- * vectors, the tridiagonal, the phasors) passes through a small LDS area
- * of the group.
+ * triangles, in registers. The coherence matrix T comes from the group's
+ * LDS (packed strict upper triangle); row-wise data every lane needs
+ * (Householder vectors, the tridiagonal, the phasors) passes through a
+ * small LDS area of the group.
  *
  *  connectivity (tcoh.cl:494-513)   bit-mask breadth-first search
  *  EVD (tcoh.cl:649-656)            eigenvector of the largest eigenvalue
@@ -27,6 +29,9 @@
  * order their LDS traffic with wavefront-scope fences, never s_barrier.
  */
 
+#ifndef TCOH_EIG_H
+#define TCOH_EIG_H
+
 #include "common.h"
 
 #define PHASOR_FLOOR 1.0e-3f
@@ -34,8 +39,10 @@
 #define EIG_FLOOR 1.0e-6f
 #define MAX_SWEEPS 40
 
+/* vw[r] = (v_r, w_r) of the current Householder step, read as one
+   16-byte word by the update. */
 template <int N> struct EigLds {
-    float2 vb[N], wb[N];
+    float4 vw[N];
     float2 tau[N];
     float d[N], e[N];
     float u0[N], u1[N], u2[N], l[N], y[N];
@@ -63,7 +70,13 @@ template <int N> __device__ __forceinline__ int lpos(int i, int j)
 
 /* Householder reduction of the group's matrix (columns in A) to the real
    tridiagonal (d, e) in LDS; the Householder vector of step k is left in
-   row k of the columns c > k (A[m][k]), tau_k in LDS. */
+   row k of the columns c > k (A[m][k]), tau_k in LDS.
+
+   Slot m holds columns m*G .. m*G+G-1; a slot whose columns are all
+   finished at step k is skipped at compile time. Finished columns inside
+   a live slot (c <= k) are updated like the others: their rows below k
+   are never read again (this layout keeps Householder vectors in rows,
+   not columns), so only the reductions and LDS writes are masked. */
 template <int N, int G, int CPL>
 __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                                                EigLds<N> &sh, int lg)
@@ -76,12 +89,15 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
         for (int m = 0; m < CPL; m++) {
             const int c = lg + m * G;
 
+            if (min(m * G + G - 1, N - 1) < k)
+                continue;
             if (c == k + 1)
                 sh.alpha = make_float2(A[m][k].x, -A[m][k].y);
             if (c == k)
                 sh.d[k] = A[m][k].x;
-            if (c >= k + 2 && c < N)
-                part += A[m][k].x * A[m][k].x + A[m][k].y * A[m][k].y;
+            const float x2 = fmaf(A[m][k].x, A[m][k].x, A[m][k].y * A[m][k].y);
+
+            part += (c >= k + 2 && c < N) ? x2 : 0.0f;
         }
         const float xn = gsum<G>(part);
         wave_sync();
@@ -91,101 +107,116 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
         if (xn == 0.0f && ali == 0.0f)
             beta = alr;
         else {
-            const float dr0 = alr, di = ali;
-
             beta = -copysignf(sqrtf(alr * alr + ali * ali + xn), alr);
             tr_ = (beta - alr) / beta;
             ti_ = -ali / beta;
             {
-                const float dr = dr0 - beta;
-                const float den = dr * dr + di * di;
+                const float dr = alr - beta;
+                const float den = dr * dr + ali * ali;
 
                 sr = dr / den;
-                si = -di / den;
+                si = -ali / den;
             }
         }
-        const float2 tau = make_float2(tr_, ti_);
         float2 vc[CPL];
+        float act[CPL];
 
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
             const int c = lg + m * G;
 
             vc[m] = make_float2(0.f, 0.f);
-            if (c == k + 1)
-                vc[m] = make_float2(1.f, 0.f);
-            if (c >= k + 2 && c < N) {
-                /* x_c = A[c][k] = conj(A[k][c]), v_c = x_c / (alpha - beta) */
-                const float2 xc = make_float2(A[m][k].x, -A[m][k].y);
+            act[m] = 0.0f;
+            if (min(m * G + G - 1, N - 1) < k + 1)
+                continue;
+            /* x_c = A[c][k] = conj(A[k][c]), v_c = x_c / (alpha - beta) */
+            const float xr = A[m][k].x, xi = -A[m][k].y;
+            const float2 xs = make_float2(fmaf(xr, sr, -xi * si),
+                                          fmaf(xr, si, xi * sr));
+            const bool live = c >= k + 1 && c < N;
 
-                vc[m] = cmul(xc, make_float2(sr, si));
-                A[m][k] = vc[m];
+            vc[m] = c == k + 1 ? make_float2(1.f, 0.f) : (live ? xs : vc[m]);
+            if (c >= k + 2)
+                A[m][k] = xs;
+            act[m] = live ? 1.0f : 0.0f;
+            if (live) {
+                sh.vw[c].x = vc[m].x;
+                sh.vw[c].y = vc[m].y;
             }
-            if (c >= k + 1 && c < N)
-                sh.vb[c] = vc[m];
         }
         if (lg == 0) {
             sh.e[k] = beta;
-            sh.tau[k] = tau;
+            sh.tau[k] = make_float2(tr_, ti_);
         }
         wave_sync();
 
-        /* w = tau A v (A Hermitian: row c is conj of column c). */
+        /* w_c = tau sum_j conj(A[j][c]) v_j (row c is conj of column c). */
         float2 wc[CPL];
         float pr = 0.0f, pi = 0.0f;
 
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
             float sr2 = 0.0f, si2 = 0.0f;
 
+            wc[m] = make_float2(0.f, 0.f);
+            if (min(m * G + G - 1, N - 1) < k + 1)
+                continue;
 #pragma unroll
             for (int j = k + 1; j < N; j++) {
-                const float2 v = sh.vb[j];
+                const float4 vw = sh.vw[j];
+                const float2 v = make_float2(vw.x, vw.y);
 
-                sr2 += A[m][j].x * v.x + A[m][j].y * v.y;
-                si2 += A[m][j].x * v.y - A[m][j].y * v.x;
+                sr2 = fmaf(A[m][j].x, v.x, sr2);
+                sr2 = fmaf(A[m][j].y, v.y, sr2);
+                si2 = fmaf(A[m][j].x, v.y, si2);
+                si2 = fmaf(-A[m][j].y, v.x, si2);
             }
-            wc[m] = cmul(tau, make_float2(sr2, si2));
-            if (c >= k + 1 && c < N) {
-                pr += wc[m].x * vc[m].x + wc[m].y * vc[m].y;
-                pi += wc[m].x * vc[m].y - wc[m].y * vc[m].x;
-            }
+            wc[m] = make_float2(fmaf(tr_, sr2, -ti_ * si2), fmaf(tr_, si2, ti_ * sr2));
+            pr = fmaf(act[m], fmaf(wc[m].x, vc[m].x, wc[m].y * vc[m].y), pr);
+            pi = fmaf(act[m], fmaf(wc[m].x, vc[m].y, -wc[m].y * vc[m].x), pi);
         }
         pr = gsum<G>(pr);
         pi = gsum<G>(pi);
         {
-            const float2 h = make_float2(-0.5f * (tr_ * pr - ti_ * pi),
-                                         -0.5f * (tr_ * pi + ti_ * pr));
+            const float hr = -0.5f * fmaf(tr_, pr, -ti_ * pi);
+            const float hi = -0.5f * fmaf(tr_, pi, ti_ * pr);
 
 #pragma unroll
             for (int m = 0; m < CPL; m++) {
                 const int c = lg + m * G;
-                const float2 hv = cmul(h, vc[m]);
 
-                wc[m].x += hv.x;
-                wc[m].y += hv.y;
-                if (c >= k + 1 && c < N)
-                    sh.wb[c] = wc[m];
+                if (min(m * G + G - 1, N - 1) < k + 1)
+                    continue;
+                wc[m].x = fmaf(hr, vc[m].x, fmaf(-hi, vc[m].y, wc[m].x));
+                wc[m].y = fmaf(hr, vc[m].y, fmaf(hi, vc[m].x, wc[m].y));
+                if (c >= k + 1 && c < N) {
+                    sh.vw[c].z = wc[m].x;
+                    sh.vw[c].w = wc[m].y;
+                }
             }
         }
         wave_sync();
 
-        /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r, c > k. */
+        /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r > k. */
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
-
-            if (c >= k + 1 && c < N) {
+            if (min(m * G + G - 1, N - 1) < k + 1)
+                continue;
 #pragma unroll
-                for (int r = k + 1; r < N; r++) {
-                    const float2 v = sh.vb[r], w = sh.wb[r];
+            for (int r = k + 1; r < N; r++) {
+                const float4 vw = sh.vw[r];
+                const float2 v = make_float2(vw.x, vw.y), w = make_float2(vw.z, vw.w);
+                float ax = A[m][r].x, ay = A[m][r].y;
 
-                    A[m][r].x -= v.x * wc[m].x + v.y * wc[m].y +
-                                 w.x * vc[m].x + w.y * vc[m].y;
-                    A[m][r].y -= v.y * wc[m].x - v.x * wc[m].y +
-                                 w.y * vc[m].x - w.x * vc[m].y;
-                }
+                ax = fmaf(-v.x, wc[m].x, ax);
+                ax = fmaf(-v.y, wc[m].y, ax);
+                ax = fmaf(-w.x, vc[m].x, ax);
+                ax = fmaf(-w.y, vc[m].y, ax);
+                ay = fmaf(-v.y, wc[m].x, ay);
+                ay = fmaf(v.x, wc[m].y, ay);
+                ay = fmaf(-w.y, vc[m].x, ay);
+                ay = fmaf(w.x, vc[m].y, ay);
+                A[m][r] = make_float2(ax, ay);
             }
         }
     }
@@ -223,11 +254,14 @@ __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
         float qv = sh.d[0] - x;
         int cnt = qv < 0.0f;
 
+        /* The count only decides a side of x: the hardware reciprocal
+           (1 ulp) is as good as a rounded division here. */
 #pragma unroll
         for (int i = 1; i < N; i++) {
             if (fabsf(qv) < pivmin)
                 qv = -pivmin;
-            qv = sh.d[i] - x - sh.e[i - 1] * sh.e[i - 1] / qv;
+            qv = sh.d[i] - x - sh.e[i - 1] * sh.e[i - 1] *
+                                   __builtin_amdgcn_rcpf(qv);
             cnt += qv < 0.0f;
         }
         const float nlo = gmax<G>(cnt <= idx ? x : lo);
@@ -591,203 +625,5 @@ __device__ void abs_inverse(const float2 *__restrict__ tp, bool est,
     wave_sync();
 }
 
-template <int N, int G, int EMI>
-__global__ __launch_bounds__(64) void k_eig(KParams kp,
-                                           const float2 *__restrict__ tbuf,
-                                           const int *__restrict__ count,
-                                           float *__restrict__ coh, int pix0,
-                                           int npix)
-{
-    constexpr int CPL = (N + G - 1) / G;
-    constexpr int PPW = WAVE / G;
-    constexpr int NT = N * (N - 1) / 2;
-    __shared__ EigLds<N> shs[PPW];
-    __shared__ EmiLds<N> ems[EMI ? PPW : 1];
-    const int gi = threadIdx.x / G, lg = threadIdx.x % G;
-    const int q = blockIdx.x * PPW + gi;
-    const bool live = q < npix;
-    const int p = pix0 + (live ? q : 0);
-    EigLds<N> &sh = shs[gi];
-    const int cnt = live ? count[p] : -1;
-    bool est = live && cnt >= kp.min_looks;
-    const float2 *tp = tbuf + (size_t)(live ? q : 0) * NT;
-    float2 A[CPL][N];
 
-#pragma unroll
-    for (int m = 0; m < CPL; m++) {
-        const int c = lg + m * G;
-
-#pragma unroll
-        for (int r = 0; r < N; r++) {
-            A[m][r] = make_float2(r == c ? 1.f : 0.f, 0.f);
-            if (est && c < N)
-                A[m][r] = load_t<N>(tp, r, c);
-        }
-    }
-
-    if (kp.bias) {
-        unsigned long long adj[CPL];
-        unsigned long long reach = 1ull;
-        const unsigned long long all = N == 64 ? ~0ull : ((1ull << N) - 1);
-
-#pragma unroll
-        for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
-
-            adj[m] = 0ull;
-#pragma unroll
-            for (int r = 0; r < N; r++)
-                if (c < N && r != c && (A[m][r].x != 0.0f || A[m][r].y != 0.0f))
-                    adj[m] |= 1ull << r;
-        }
-        for (int it = 0; it < N; it++) {
-            unsigned long long add = 0ull;
-
-#pragma unroll
-            for (int m = 0; m < CPL; m++) {
-                const int c = lg + m * G;
-
-                if (c < N && ((reach >> c) & 1ull))
-                    add |= adj[m];
-            }
-            const unsigned long long nr = reach | gor64<G>(add);
-
-            if (nr == reach)
-                break;
-            reach = nr;
-        }
-        est = est && reach == all;
-    }
-
-    float2 z[CPL];
-
-    if (!EMI) {
-        hermitian_eigenvector<N, G, CPL>(A, sh, N - 1, lg, z);
-    }
-    else {
-        float inv[CPL][N];
-
-        abs_inverse<N, G, CPL>(tp, est, ems[EMI ? gi : 0], lg, inv);
-        /* M = |T|^-1 o T, reloaded so that T need not stay in registers
-           through the inversion. */
-#pragma unroll
-        for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
-
-#pragma unroll
-            for (int r = 0; r < N; r++) {
-                float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
-
-                if (est && c < N)
-                    t = load_t<N>(tp, r, c);
-                A[m][r] = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
-            }
-        }
-        hermitian_eigenvector<N, G, CPL>(A, sh, 0, lg, z);
-    }
-
-    /* Phasors relative to date N/2 or to the last strongest component. */
-    float mags[CPL], lmax = 0.0f;
-
-#pragma unroll
-    for (int m = 0; m < CPL; m++) {
-        const int c = lg + m * G;
-
-        mags[m] = hypotf(z[m].x, z[m].y);
-        if (c < N) {
-            lmax = fmaxf(lmax, mags[m]);
-            sh.z[c] = z[m];
-        }
-    }
-    lmax = gmax<G>(lmax);
-    wave_sync();
-    int ref = N / 2;
-    {
-        const float2 zr = sh.z[N / 2];
-
-        if (hypotf(zr.x, zr.y) <= PHASOR_FLOOR * lmax) {
-            int last = -1;
-
-#pragma unroll
-            for (int m = 0; m < CPL; m++) {
-                const int c = lg + m * G;
-
-                if (c < N && mags[m] == lmax)
-                    last = c;
-            }
-            ref = gmaxi<G>(last);
-            if (ref < 0) /* only with non-finite magnitudes */
-                ref = N / 2;
-        }
-    }
-    const float2 zref = sh.z[ref];
-    const float refm = hypotf(zref.x, zref.y);
-    const float refr = zref.x / refm, refi = -zref.y / refm;
-    const float floor_ = lmax * PHASOR_FLOOR;
-    wave_sync();
-#pragma unroll
-    for (int m = 0; m < CPL; m++) {
-        const int c = lg + m * G;
-        const float re = z[m].x * refr - z[m].y * refi;
-        const float im = z[m].x * refi + z[m].y * refr;
-        const float mg = hypotf(re, im);
-
-        z[m] = mg > floor_ ? make_float2(re / mg, im / mg) : make_float2(1.f, 0.f);
-        if (c < N)
-            sh.z[c] = z[m];
-    }
-    wave_sync();
-
-    /* gamma: |sum_{i<j} unit(T_ij) conj(z_i) z_j| / (N(N-1)/2) */
-    float gr = 0.0f, gim = 0.0f;
-
-#pragma unroll
-    for (int m = 0; m < CPL; m++) {
-        const int c = lg + m * G;
-
-        if (est && c < N) {
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                if (i < c) {
-                    const float2 t = tp[upper_index(N, i, c)];
-                    const float mg = hypotf(t.x, t.y);
-                    const float ur = mg > 0.0f ? t.x / mg : 1.0f;
-                    const float ui = mg > 0.0f ? t.y / mg : 0.0f;
-                    const float2 w = cmulc(sh.z[i], z[m]);
-
-                    gr += ur * w.x - ui * w.y;
-                    gim += ur * w.y + ui * w.x;
-                }
-            }
-        }
-    }
-    gr = gsum<G>(gr);
-    gim = gsum<G>(gim);
-    if (live && lg == 0)
-        coh[p] = est ? hypotf(gr, gim) / (0.5f * N * (N - 1))
-                     : __builtin_nanf("");
-}
-
-template <int N>
-hipError_t launch_eig_n(const KParams &kp, const float2 *tbuf,
-                        const int *count, float *coh, int pix0, int npix,
-                        hipStream_t s)
-{
-    constexpr int G = N <= 16 ? 8 : (N <= 32 ? 16 : 64);
-    constexpr int PPW = WAVE / G;
-    const dim3 grid((npix + PPW - 1) / PPW);
-
-    if (kp.emi)
-        hipLaunchKernelGGL((k_eig<N, G, 1>), grid, dim3(WAVE), 0, s, kp, tbuf,
-                           count, coh, pix0, npix);
-    else
-        hipLaunchKernelGGL((k_eig<N, G, 0>), grid, dim3(WAVE), 0, s, kp, tbuf,
-                           count, coh, pix0, npix);
-    return hipGetLastError();
-}
-
-/* One translation unit per stack depth (the build compiles this file once
-   for every N, in parallel). */
-template hipError_t launch_eig_n<TCOH_EIG_N>(const KParams &, const float2 *,
-                                             const int *, float *, int, int,
-                                             hipStream_t);
+#endif

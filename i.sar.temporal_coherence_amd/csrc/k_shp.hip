@@ -51,8 +51,8 @@ __global__ __launch_bounds__(256) void k_prep(KParams kp,
 
     if (pix >= kp.npad)
         return;
-    for (int k = 0; k < n; k++)
-        ampT[(size_t)k * kp.npad + pix] = amp[(size_t)pix * n + k];
+    for (int k = 0; k < shp_bucket(n); k++)
+        ampT[(size_t)k * kp.npad + pix] = k < n ? amp[(size_t)pix * n + k] : 0.0f;
     if (kp.test == 2) {
         float mean = 0.0f, var = 0.0f;
 
@@ -71,21 +71,34 @@ __global__ __launch_bounds__(256) void k_prep(KParams kp,
     }
 }
 
+/* With ahi[j] = a[j+K] and alo[j] = a[j-K] (the centre shifted by K,
+   loaded once per pixel): accept iff b[j] < ahi[j] for j < N-K and
+   alo[j] <= b[j] for j >= K. All N candidate samples are loaded
+   unconditionally (clamped index) so the loads issue back to back. */
+/* ampT holds NB planes (n..NB-1 zero), so plane j of a candidate is
+   ampT[cand + j * npad] for every j < NB; npv is npad in a vector
+   register, so the 32-bit plane offsets are per-lane adds. */
 template <int NB>
-__device__ __forceinline__ bool ks_ok(const float (&a)[NB], int n, int kmax,
+__device__ __forceinline__ bool ks_ok(const float (&ahi)[NB],
+                                      const float (&alo)[NB], int n, int kmax,
                                       const float *__restrict__ ampT,
-                                      int npad, int cand)
+                                      int cand, int npv)
 {
+    float b[NB];
     bool ok = kmax >= 1;
+    int idx = cand;
 
 #pragma unroll
-    for (int i = 0; i < NB; i++)
-        if (i >= kmax && i < n)
-            ok &= ampT[(size_t)(i - kmax) * npad + cand] < a[i];
+    for (int j = 0; j < NB; j++) {
+        b[j] = ampT[idx];
+        idx += npv;
+    }
+    /* Dates without a condition carry NaN thresholds: !(b >= NaN) and
+       !(NaN > b) hold, and for numbers they are b < ahi and alo <= b
+       (amplitudes of valid pixels are never NaN). */
 #pragma unroll
     for (int j = 0; j < NB; j++)
-        if (j + kmax < n)
-            ok &= a[j] <= ampT[(size_t)(j + kmax) * npad + cand];
+        ok &= !(b[j] >= ahi[j]) & !(alo[j] > b[j]);
     return ok;
 }
 
@@ -100,27 +113,50 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
 {
     /* AD walks the centre sample with a data-dependent index. */
     __shared__ float acol[TEST == 1 ? NB : 1][WAVE];
-    const int out = blockIdx.x * WAVE + threadIdx.x;
     const int n = kp.n;
-    const int y = out / kp.cols, x = out - (out / kp.cols) * kp.cols;
+    int y, tx;
+
+    if (!xcd_tile(blockIdx.x, kp.rows, (kp.cols + WAVE - 1) / WAVE, &y, &tx))
+        return;
+    const int x = tx * WAVE + threadIdx.x;
+
+    if (x >= kp.cols)
+        return;
+    const int out = y * kp.cols + x;
     const int centre = (y + kp.wa / 2) * kp.w + x + kp.wr / 2;
     uint32_t *mk = mask + (size_t)out * kp.words;
-    float a[NB];
+    float a[NB], ahi[NB], alo[NB];
     float mean_a = 0.0f, var_a = 0.0f;
     uint32_t word = 0;
     int bit = 0, wi = 0, looks = 0;
 
-    if (out >= kp.rows * kp.cols)
-        return;
     if (!valid[centre]) {
         count[out] = -1;
         for (int i = 0; i < kp.words; i++)
             mk[i] = 0u;
         return;
     }
+    const int npv = in_vgpr(kp.npad);
+
 #pragma unroll
     for (int i = 0; i < NB; i++)
-        a[i] = i < n ? ampT[(size_t)i * kp.npad + centre] : 0.0f;
+        a[i] = ampT[centre + i * npv];
+
+    if (TEST == 0) {
+        /* ahi[j] = a[j+K] for j < N-K, alo[j] = a[j-K] for K <= j < N,
+           NaN where the reference walk has no such condition */
+        const int k = kp.ks_max;
+        const float nan = __builtin_nanf("");
+
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const float h = ampT[centre + min(j + k, n - 1) * npv];
+            const float l = ampT[centre + min(max(j - k, 0), n - 1) * npv];
+
+            ahi[j] = j + k < n ? h : nan;
+            alo[j] = (j >= k && j < n) ? l : nan;
+        }
+    }
     if (TEST == 1) {
 #pragma unroll
         for (int i = 0; i < NB; i++)
@@ -141,11 +177,12 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
             const int cand = row + dx;
             bool ok = valid[cand] != 0;
 
-            if (ok) {
-                if (TEST == 0) {
-                    ok = ks_ok<NB>(a, n, kp.ks_max, ampT, kp.npad, cand);
-                }
-                else if (TEST == 1) {
+            if (TEST == 0) {
+                /* every load is in bounds: test unconditionally */
+                ok = ks_ok<NB>(ahi, alo, n, kp.ks_max, ampT, cand, npv) & ok;
+            }
+            else if (ok) {
+                if (TEST == 1) {
                     int ia = 0, ib = 0, m = 0;
                     float s = 0.0f;
 
@@ -204,7 +241,7 @@ static void launch_shp_nb(const KParams &kp, const float *ampT,
                           const float *adtab, uint32_t *mask, int *count,
                           hipStream_t s)
 {
-    const dim3 grid((kp.rows * kp.cols + WAVE - 1) / WAVE);
+    const dim3 grid(xcd_grid(kp.rows, (kp.cols + WAVE - 1) / WAVE));
 
     if (kp.test == 0)
         hipLaunchKernelGGL((k_shp<NB, 0>), grid, dim3(WAVE), 0, s, kp, ampT,

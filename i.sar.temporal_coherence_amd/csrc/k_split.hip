@@ -1,0 +1,112 @@
+/* Preparation of the mixed-precision covariance: every sample component x
+ * (after a common power-of-two scale of the strip) becomes a 32-bit word
+ * hi | lo << 16 with hi = round16(x), lo = round16(x - hi), FP16 or BF16,
+ * round to nearest even. One zero pixel is appended (index npad) for the
+ * padding of candidate lists. */
+
+#include "common.h"
+
+__device__ __forceinline__ uint32_t f16_bits(float x)
+{
+    const _Float16 h = (_Float16)x;
+    unsigned short u;
+
+    __builtin_memcpy(&u, &h, 2);
+    return u;
+}
+
+__device__ __forceinline__ float f16_value(uint32_t u)
+{
+    const unsigned short s = (unsigned short)u;
+    _Float16 h;
+
+    __builtin_memcpy(&h, &s, 2);
+    return (float)h;
+}
+
+/* Round to nearest even BF16 (finite inputs). */
+__device__ __forceinline__ uint32_t bf16_bits(float x)
+{
+    const uint32_t b = __float_as_uint(x);
+
+    return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+}
+
+__device__ __forceinline__ float bf16_value(uint32_t u)
+{
+    return __uint_as_float(u << 16);
+}
+
+__global__ __launch_bounds__(256) void k_split(KParams kp,
+                                              const float2 *__restrict__ slc,
+                                              uint2 *__restrict__ hl, int bf16,
+                                              float scale)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = ((size_t)kp.npad + 1) * kp.n;
+
+    if (i >= total)
+        return;
+    const float2 v = i < (size_t)kp.npad * kp.n ? slc[i] : make_float2(0.f, 0.f);
+    const float c[2] = {v.x * scale, v.y * scale};
+    uint32_t w[2];
+
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        uint32_t h, l;
+
+        if (bf16) {
+            h = bf16_bits(c[t]);
+            l = bf16_bits(c[t] - bf16_value(h));
+        }
+        else {
+            h = f16_bits(c[t]);
+            l = f16_bits(c[t] - f16_value(h));
+        }
+        w[t] = h | (l << 16);
+    }
+    hl[i] = make_uint2(w[0], w[1]);
+}
+
+/* Largest |re|, |im| of the strip as float bits (non-negative floats
+   order like their bit patterns). */
+__global__ __launch_bounds__(256) void k_absmax(const float2 *__restrict__ slc,
+                                               size_t n,
+                                               unsigned int *__restrict__ out)
+{
+    unsigned int m = 0;
+
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * 256) {
+        const float2 v = slc[i];
+        const float a = fmaxf(fabsf(v.x), fabsf(v.y));
+
+        if (isfinite(a))
+            m = max(m, __float_as_uint(a));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(out, m);
+}
+
+__global__ void k_zero_u32(unsigned int *out) { *out = 0u; }
+
+hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
+                        int bf16, float scale, hipStream_t s)
+{
+    const size_t total = ((size_t)kp.npad + 1) * kp.n;
+
+    hipLaunchKernelGGL(k_split, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                       0, s, kp, slc, (uint2 *)hl, bf16, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float2 *slc, size_t n, unsigned int *out,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, out);
+    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, slc, n, out);
+    return hipGetLastError();
+}

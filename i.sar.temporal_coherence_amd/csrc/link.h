@@ -1,0 +1,180 @@
+/* Linking stage of one pixel per group of G lanes: connectivity, EVD or
+ * EMI phase linking, phasors and the temporal coherence (tcoh.cl:641-744),
+ * from the pixel's coherence matrix T in the group's LDS (packed strict
+ * upper triangle, unit diagonal implied).
+ */
+#ifndef TCOH_LINK_H
+#define TCOH_LINK_H
+
+#include "eig.h"
+
+/* gamma of the group's pixel, NaN when it is not estimated or (with the
+   bias correction) its dates are not linked. */
+template <int N, int G, int EMI>
+__device__ __forceinline__ float link_group(const KParams &kp,
+                                            const float2 *tp, bool est,
+                                            int lg, EigLds<N> &sh,
+                                            EmiLds<N> &em)
+{
+    constexpr int CPL = (N + G - 1) / G;
+    float2 A[CPL][N];
+
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+        const int c = lg + m * G;
+
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            A[m][r] = make_float2(r == c ? 1.f : 0.f, 0.f);
+            if (est && c < N)
+                A[m][r] = load_t<N>(tp, r, c);
+        }
+    }
+
+    /* Breadth-first search over the non-zero coherences (tcoh.cl:494-513,
+       641-647), one bit per date. */
+    if (kp.bias) {
+        unsigned long long adj[CPL];
+        unsigned long long reach = 1ull;
+        const unsigned long long all = N == 64 ? ~0ull : ((1ull << N) - 1);
+
+#pragma unroll
+        for (int m = 0; m < CPL; m++) {
+            const int c = lg + m * G;
+
+            adj[m] = 0ull;
+#pragma unroll
+            for (int r = 0; r < N; r++)
+                if (c < N && r != c && (A[m][r].x != 0.0f || A[m][r].y != 0.0f))
+                    adj[m] |= 1ull << r;
+        }
+        for (int it = 0; it < N; it++) {
+            unsigned long long add = 0ull;
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++) {
+                const int c = lg + m * G;
+
+                if (c < N && ((reach >> c) & 1ull))
+                    add |= adj[m];
+            }
+            const unsigned long long nr = reach | gor64<G>(add);
+
+            if (nr == reach)
+                break;
+            reach = nr;
+        }
+        est = est && reach == all;
+    }
+
+    float2 z[CPL];
+
+    if (!EMI) {
+        hermitian_eigenvector<N, G, CPL>(A, sh, N - 1, lg, z);
+    }
+    else {
+        float inv[CPL][N];
+
+        abs_inverse<N, G, CPL>(tp, est, em, lg, inv);
+        /* M = |T|^-1 o T, reloaded so that T need not stay in registers
+           through the inversion. */
+#pragma unroll
+        for (int m = 0; m < CPL; m++) {
+            const int c = lg + m * G;
+
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
+
+                if (est && c < N)
+                    t = load_t<N>(tp, r, c);
+                A[m][r] = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
+            }
+        }
+        hermitian_eigenvector<N, G, CPL>(A, sh, 0, lg, z);
+    }
+
+    /* Phasors relative to date N/2 or to the last strongest component
+       (tcoh.cl:696-724). */
+    float mags[CPL], lmax = 0.0f;
+
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+        const int c = lg + m * G;
+
+        mags[m] = hypotf(z[m].x, z[m].y);
+        if (c < N) {
+            lmax = fmaxf(lmax, mags[m]);
+            sh.z[c] = z[m];
+        }
+    }
+    lmax = gmax<G>(lmax);
+    wave_sync();
+    int ref = N / 2;
+    {
+        const float2 zr = sh.z[N / 2];
+
+        if (hypotf(zr.x, zr.y) <= PHASOR_FLOOR * lmax) {
+            int last = -1;
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++) {
+                const int c = lg + m * G;
+
+                if (c < N && mags[m] == lmax)
+                    last = c;
+            }
+            ref = gmaxi<G>(last);
+            if (ref < 0) /* only with non-finite magnitudes */
+                ref = N / 2;
+        }
+    }
+    const float2 zref = sh.z[ref];
+    const float refm = hypotf(zref.x, zref.y);
+    const float refr = zref.x / refm, refi = -zref.y / refm;
+    const float floor_ = lmax * PHASOR_FLOOR;
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+        const int c = lg + m * G;
+        const float re = z[m].x * refr - z[m].y * refi;
+        const float im = z[m].x * refi + z[m].y * refr;
+        const float mg = hypotf(re, im);
+
+        z[m] = mg > floor_ ? make_float2(re / mg, im / mg) : make_float2(1.f, 0.f);
+        if (c < N)
+            sh.z[c] = z[m];
+    }
+    wave_sync();
+
+    /* gamma = |sum_{i<j} unit(T_ij) conj(z_i) z_j| / (N(N-1)/2), unit(0)
+       = 1 (tcoh.cl:726-744); lane c sums its column. */
+    float gr = 0.0f, gim = 0.0f;
+
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+        const int c = lg + m * G;
+
+        if (!est || m * G >= N)
+            continue;
+#pragma unroll 4
+        for (int i = 0; i < N - 1; i++) {
+            const bool up = i < c && c < N;
+            const float2 t = tp[up ? upper_index(N, i, c) : 0];
+            const float m2 = fmaf(t.x, t.x, t.y * t.y);
+            const float rm = __builtin_amdgcn_rsqf(m2);
+            const float ur = m2 > 0.0f ? t.x * rm : 1.0f;
+            const float ui = m2 > 0.0f ? t.y * rm : 0.0f;
+            const float2 w = cmulc(sh.z[i], z[m]);
+            const float wt = up ? 1.0f : 0.0f;
+
+            gr = fmaf(wt, fmaf(ur, w.x, -ui * w.y), gr);
+            gim = fmaf(wt, fmaf(ur, w.y, ui * w.x), gim);
+        }
+    }
+    gr = gsum<G>(gr);
+    gim = gsum<G>(gim);
+    return est ? hypotf(gr, gim) / (0.5f * N * (N - 1)) : __builtin_nanf("");
+}
+
+#endif
