@@ -267,8 +267,8 @@ __device__ __forceinline__ void cov_mfma_pixel(const KParams &kp,
     for (int t = 0; t < NBK * NBK; t++)
         acc_ri[t] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-    /* Gathers of chunk c+1 are issued before the MFMAs of chunk c. */
-    uint2 cur[NBK][8], nxt[NBK][8];
+    /* Gathers run two chunks ahead of the MFMAs. */
+    uint2 cur[NBK][8], nxt[NBK][8], nx2[NBK][8];
     auto fetch = [&](int c0, uint2(&v)[NBK][8]) {
         int idx[8];
 
@@ -286,9 +286,11 @@ __device__ __forceinline__ void cov_mfma_pixel(const KParams &kp,
     };
 
     fetch(0, cur);
+    if (32 < lpad)
+        fetch(32, nxt);
     for (int c0 = 0; c0 < lpad; c0 += 32) {
-        if (c0 + 32 < lpad)
-            fetch(c0 + 32, nxt);
+        if (c0 + 64 < lpad)
+            fetch(c0 + 64, nx2);
         Opnd op[NBK];
 
 #pragma unroll
@@ -297,8 +299,10 @@ __device__ __forceinline__ void cov_mfma_pixel(const KParams &kp,
 #pragma unroll
         for (int bk = 0; bk < NBK; bk++)
 #pragma unroll
-            for (int e = 0; e < 8; e++)
+            for (int e = 0; e < 8; e++) {
                 cur[bk][e] = nxt[bk][e];
+                nxt[bk][e] = nx2[bk][e];
+            }
         int t = 0;
 #pragma unroll
         for (int a = 0; a < NBK; a++)

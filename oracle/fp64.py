@@ -83,6 +83,23 @@ def coherence(c, looks, bias):
     return t
 
 
+def bias_m2c(c, looks):
+    """Debiased squared magnitudes m2c = (L m2 - 1) / (L - 1) (tcoh.cl:620-633).
+
+    The bias correction zeroes T_ij where m2c <= 0 and otherwise gives
+    |T_ij| = sqrt(m2c); gamma counts a zero T_ij as unit(0) = 1. An entry
+    whose m2c lies within a form's rounding of zero is therefore a threshold
+    decision on which gamma may differ by up to 2 / (N(N-1)/2) between
+    correct implementations, and sqrt(m2c) is steep there.
+    """
+    d = np.sqrt(np.maximum(np.real(np.einsum("pii->pi", c)), 0.0))
+    den = d[:, :, None] * d[:, None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        m2 = np.where(den > 0, np.abs(c) ** 2 / np.where(den > 0, den, 1.0) ** 2, 0.0)
+    lk = looks[:, None, None].astype(np.float64)
+    return (lk * m2 - 1.0) / np.maximum(lk - 1.0, 1.0)
+
+
 def connected(t):
     """Whether the non-zero coherences link all dates (tcoh.cl:494-513)."""
     n = t.shape[-1]
@@ -176,7 +193,7 @@ def estimate(t, emi, loading=0.0):
 
 
 def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
-        chunk=4096, emi_loading=0.0):
+        chunk=4096, emi_loading=0.0, band=None):
     """FP64 coherence and pair coherences of the strip's output pixels.
 
     slc: float32 (npad, N, 2) or complex; count: int (rows*cols,) from the
@@ -184,7 +201,10 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
     settings: dict with ndates, win_az, win_rg, min_looks, bias, emi,
     npairs, pairs_all. pixels: flat output indices to evaluate (default
     all). Returns (coh (len(pixels),), pairs (len(pixels), npairs)) with NaN
-    where not estimated, like the kernel.
+    where not estimated, like the kernel. With band (bias correction only)
+    also returns near (len(pixels),): the number of pairs i < j with
+    |m2c| < band, and m2c (len(pixels), npairs) of the output pairs (NaN
+    where not estimated).
     """
     n = settings["ndates"]
     wa, wr = settings["win_az"], settings["win_rg"]
@@ -203,6 +223,8 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
             pi, pj = np.arange(n - 1), np.arange(1, n)
     coh = np.full(len(pixels), np.nan)
     pairs = np.full((len(pixels), npairs), np.nan)
+    near = np.zeros(len(pixels), np.int64)
+    pm2c = np.full((len(pixels), npairs), np.nan)
     for lo in range(0, len(pixels), chunk):
         sel = pixels[lo:lo + chunk]
         cnt = count[sel]
@@ -216,10 +238,18 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
         c = covariance(slc, acc, origin, offs)
         t = coherence(c, cnt[est], settings["bias"])
         out = np.flatnonzero(est) + lo
+        if band is not None and settings["bias"]:
+            m2c = bias_m2c(c, cnt[est])
+            up = np.triu(np.ones((n, n), bool), 1)
+            near[out] = np.sum((np.abs(m2c) < band) & up, axis=(1, 2))
+            if npairs:
+                pm2c[out] = m2c[:, pi, pj]
         if npairs:
             pairs[out] = np.abs(t[:, pi, pj])
         ok = connected(t) if settings["bias"] else np.ones(len(p), bool)
         if ok.any():
             z = estimate(t[ok], settings["emi"], emi_loading)
             coh[out[ok]] = gamma(t[ok], z)
+    if band is not None:
+        return coh, pairs, near, pm2c
     return coh, pairs

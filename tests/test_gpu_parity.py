@@ -15,14 +15,17 @@ pytestmark = pytest.mark.gpu
 
 PRECISIONS = ["plain", "f16", "bf16"]
 
-# The BF16 hi/lo split keeps ~17 bits of each sample (FP16: ~22), which the
-# bias correction with few looks amplifies past 1e-4: the BF16 form is held
-# to this looser bound, the plain and FP16 forms to TOLERANCE.
-BF16_TOLERANCE = 1e-3
+# Bias-correction threshold band of each form: a bound on the error its
+# rounding makes in m2c = (L m2 - 1) / (L - 1) (tcoh.cl:620-633). The BF16
+# split keeps ~2^-18 of each sample, so |dT| <~ 3e-5 and |dm2c| <~
+# 2 |T| |dT| L / (L - 1) ~ 1e-5 where |T|^2 ~ 1/L; FP16 keeps 2^-22 (~7e-7);
+# plain FP32 sums stay below 1e-6. (With the splits emulated in double on
+# the N=30 crop below: 1.7e-6 BF16, 1.9e-7 FP16.)
+M2C_BAND = {"plain": 1e-6, "f16": 1e-6, "bf16": 1e-5}
 
 
 def tol_for(precision):
-    return BF16_TOLERANCE if precision == "bf16" else TOLERANCE
+    return TOLERANCE
 
 
 def settings_for(tcoh, n, opts):
@@ -46,13 +49,29 @@ def check_against_oracles(tcoh, cref, fp64, stack, opts, precision, tol=TOLERANC
     with tcoh.Backend(s, pcols, precision=precision) as be:
         coh, count, pairs = be.run(slc, amp, valid, prow, rows, cols)
     assert np.array_equal(count, ocount), "SHP counts differ from the reference kernel"
-    c64, p64 = fp64.run(slc, ocount, mask, cref.settings_dict(so), pcols, rows, cols)
-    assert np.array_equal(np.isnan(coh), np.isnan(c64))
-    err = np.nanmax(np.abs(coh - c64)) if np.isfinite(c64).any() else 0.0
-    assert err <= tol, "max |d gamma| %.3g" % err
+    band = M2C_BAND[precision]
+    c64, p64, near, m2c = fp64.run(slc, ocount, mask, cref.settings_dict(so), pcols,
+                                   rows, cols, band=band)
+    # With bias correction an entry within the band of m2c = 0 is a
+    # threshold decision: T_ij = 0 there counts as unit(0) = 1 in gamma
+    # (2 / (N(N-1)/2) per entry) and may cut the coherence graph; pair
+    # coherences sqrt(m2c) are steep there. Everywhere else the bar is tol.
+    edge = near > 0
+    assert np.array_equal(np.isnan(coh)[~edge], np.isnan(c64)[~edge])
+    both = np.isfinite(coh) & np.isfinite(c64)
+    allow = tol + 2.0 * near / (0.5 * n * (n - 1))
+    err = np.abs(coh - c64)
+    worst = np.argmax(np.where(both, err - allow, -np.inf)) if both.any() else 0
+    assert not both.any() or err[worst] <= allow[worst], \
+        "max |d gamma| %.3g at pixel %d (allowed %.3g)" % (err[worst], worst, allow[worst])
     assert np.array_equal(np.isnan(pairs.T), np.isnan(p64))
     if np.isfinite(p64).any():
-        assert np.nanmax(np.abs(pairs.T - p64)) <= tol
+        fin = np.isfinite(m2c)
+        m = np.where(fin, m2c, 0.0)
+        pallow = tol + np.where(fin, np.sqrt(np.maximum(m + band, 0)) -
+                                np.sqrt(np.maximum(m - band, 0)), 0.0)
+        perr = np.abs(pairs.T - p64)
+        assert np.nanmax(perr - pallow) <= 0, "max |d pair| %.3g" % np.nanmax(perr)
     return coh, count, pairs
 
 

@@ -259,6 +259,46 @@ def test_bias_unlinked_dates_give_nan_with_finite_pairs(cref, fp64, tcoh):
     assert np.isfinite(pairs[0][unlinked]).all()
 
 
+def test_bias_pairs_are_sqrt_m2c(cref, fp64, tcoh):
+    """Debiased |T_ij| = sqrt(max(m2c, 0)) (tcoh.cl:620-633): the closed form
+    the parity tests use to bound pair errors near the threshold."""
+    case = CASES[3]
+    n, opts = case["n"], case["opts"]
+    stack = tcoh.synth.ds_stack(n, case["g"], seed=case["seed"])
+    s = cref.settings(n, window=opts["window"], bias=True, min_shp=opts["min_shp"],
+                      pairs="all")
+    slc, amp, valid, prow, pcols = strip_of(stack, opts["window"])
+    rows, cols = stack.shape[1:]
+    _c, count, _p, mask, _f = cref.run(s, pcols, slc, amp, valid, prow, rows, cols)
+    _c64, p64, near, m2c = fp64.run(slc, count, mask, cref.settings_dict(s), pcols,
+                                    rows, cols, band=1e-3)
+    fin = np.isfinite(p64)
+    assert fin.any() and (m2c[fin] <= 0).any() and (m2c[fin] > 0).any()
+    assert np.allclose(p64[fin], np.sqrt(np.maximum(m2c[fin], 0)), rtol=0, atol=1e-12)
+    assert np.array_equal(near, np.sum(np.abs(np.nan_to_num(m2c, nan=1.0)) < 1e-3, axis=1))
+
+
+def test_gamma_step_at_a_zeroed_coherence(fp64):
+    """A coherence that the bias correction zeroes counts as unit(0) = 1 in
+    gamma (tcoh.cl:726-744): moving one entry from a tiny value to zero
+    changes gamma by at most 2 / (N(N-1)/2), the allowance of the parity
+    tests at the threshold."""
+    rng = np.random.default_rng(8)
+    n = 12
+    z = np.exp(1j * rng.uniform(-np.pi, np.pi, n))
+    t = 0.6 * np.outer(z, z.conj()) + 0.4 * np.eye(n)
+    worst = 0.0
+    for i, j in [(0, 1), (3, 7), (10, 11)]:
+        for ph in np.linspace(-np.pi, np.pi, 9):
+            a, b = t.copy(), t.copy()
+            a[i, j] = 1e-9 * np.exp(1j * ph)
+            b[i, j] = 0.0
+            a[j, i], b[j, i] = np.conj(a[i, j]), 0.0
+            g = fp64.gamma(np.stack([a, b]), np.stack([z, z]))
+            worst = max(worst, abs(g[0] - g[1]))
+    assert 1.0 / (n * (n - 1) / 2) < worst <= 2.0 / (n * (n - 1) / 2) + 1e-12
+
+
 def test_coherent_and_incoherent(cref, fp64, tcoh):
     """reference test:138-151."""
     means = {}
