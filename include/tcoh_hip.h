@@ -20,6 +20,11 @@
 extern "C" {
 #endif
 
+/* A C file that already has the reference's types (it includes
+   local_proto.h) defines TCOH_HAVE_REFERENCE_TYPES before this header and
+   takes only the entry points; the layouts are the same. */
+#ifndef TCOH_HAVE_REFERENCE_TYPES
+
 /* local_proto.h:9-10: largest supported stack depth. */
 #define MAX_DATES 64
 
@@ -61,6 +66,8 @@ struct geometry {
     double *phase; /* [node][height][date] */
     double h0;     /* height used without elevation map */
 };
+
+#endif /* TCOH_HAVE_REFERENCE_TYPES */
 
 /* stats.c:61-95: KS_MAX, AD_NORM, AD_SIGMA, AD_CRIT, TLOG_CRIT from
    s->ndates, s->alpha and s->test. */
@@ -157,10 +164,11 @@ void tcoh_run_device(struct ocl *o, const void *slc, const void *amp,
                      const void *valid, int padded_rows, int rows, int cols,
                      void *coh, void *count, void *pairs, void *stream);
 
-/* Device time of the last run's stages in milliseconds (prep incl. the
-   hi/lo split, shp, covariance, eigen), summed over the stage's kernel
-   launches, measured with HIP events on the run's stream when timing is
-   on; and the number of launches of each stage. */
+/* Device time of the last run's stages in milliseconds: [0] preparation
+   (planar amplitudes, and the hi/lo split in the mixed forms), [1] SHP
+   selection, [2] the fused covariance + linking kernel, [3] reserved (0);
+   summed over the stage's kernel launches, measured with HIP events on the
+   run's stream when timing is on; and the number of launches of each. */
 void tcoh_set_timing(struct ocl *o, int on);
 void tcoh_stage_times(const struct ocl *o, double ms[4]);
 void tcoh_stage_launches(const struct ocl *o, int launches[4]);

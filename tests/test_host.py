@@ -29,6 +29,56 @@ def test_library_exports_every_declared_symbol(tcoh):
         assert hasattr(lib, name), name
 
 
+C_CALLER = r"""
+#include <stdio.h>
+#include "tcoh_hip.h"
+
+int main(void)
+{
+    struct settings s = {0};
+
+    s.ndates = 30;
+    s.alpha = 0.05;
+    s.test = SHP_KS;
+    shp_constants(&s);
+    printf("%d %.9e\n", s.ks_max, s.tlog_crit);
+    return 0;
+}
+"""
+
+# A caller that has the reference's types from its own header.
+C_REFERENCE_TYPES = r"""
+struct settings;
+struct orbit;
+struct geometry;
+#define TCOH_HAVE_REFERENCE_TYPES
+#include "tcoh_hip.h"
+void caller(struct ocl *o) { ocl_close(o); }
+"""
+
+
+def test_c_caller_links_against_the_library(tcoh, tmp_path):
+    """A C program built against include/tcoh_hip.h links and calls the
+    library as main.c would (no GPU call)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    lib = os.path.dirname(tcoh.api.LIB_PATH)
+    src, exe = tmp_path / "caller.c", tmp_path / "caller"
+    src.write_text(C_CALLER)
+    subprocess.run([gcc, "-std=c11", "-Wall", "-Werror", str(src), "-I",
+                    os.path.join(ROOT, "include"), "-L", lib, "-ltcoh_hip",
+                    "-Wl,-rpath," + lib, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.split() == ["10", "1.959963985e+00"]
+    ref = tmp_path / "reftypes.c"
+    ref.write_text(C_REFERENCE_TYPES)
+    subprocess.run([gcc, "-std=c11", "-Wall", "-Werror", "-fsyntax-only", str(ref),
+                    "-I", os.path.join(ROOT, "include")], check=True)
+
+
 def test_settings_layout_matches_reference_struct(tcoh, cref):
     """struct settings of local_proto.h:33-49 in both bindings."""
     a, b = tcoh.api.Settings, cref.OSettings
