@@ -89,33 +89,66 @@ def run_form(tcoh, s, pcols, bufs, rows, cols, precision, steps, warmup, torch):
     return ms_step, stage_ms / steps, launches / steps
 
 
+def pixel_conditioning(fp64, slc_c, count, mask, sd, pcols, size, k):
+    """Why crop pixel k may differ: in double, its smallest |m2c| (bias
+    threshold), the relative gap of T's two largest eigenvalues, and its
+    smallest phasor magnitude in units of the 1e-3 floor."""
+    n, wa, wr = sd["ndates"], sd["win_az"], sd["win_rg"]
+    y, x = divmod(int(k), size)
+    acc = fp64.mask_bits(mask[[k]], wa * wr)
+    c = fp64.covariance(slc_c, acc, np.array([y * pcols + x]),
+                        fp64.window_offsets(wa, wr, pcols))
+    t = fp64.coherence(c, count[[k]], sd["bias"])
+    w, v = np.linalg.eigh(t)
+    mag = np.abs(v[0, :, -1])
+    up = np.triu(np.ones((n, n), bool), 1)
+    out = {"pixel": [y, x], "count": int(count[k]),
+           "top_eig_gap": float((w[0, -1] - w[0, -2]) / w[0, -1]),
+           "min_phasor_over_floor": float(np.min(mag) / (1e-3 * mag.max()))}
+    if sd["bias"]:
+        out["min_abs_m2c"] = float(np.min(np.abs(fp64.bias_m2c(c, count[[k]])[0][up])))
+    return out
+
+
 def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, s_opts, y0, x0, size,
                   gpu_maps):
-    """Oracle SHP counts and FP64 gamma/pairs on a size x size crop."""
+    """Oracle SHP counts and FP64 gamma/pairs on a size x size crop; the
+    FP32 C oracle's own gamma error beside the GPU forms'."""
     n, wa, wr = 30, 21, 7
     pr_c, pc_c = size + wa - 1, size + wr - 1
     slc3 = slc.reshape(-1, pcols, n * 2)[y0:y0 + pr_c, x0:x0 + pc_c]
     amp3 = amp.reshape(-1, pcols, n)[y0:y0 + pr_c, x0:x0 + pc_c]
     val3 = valid.reshape(-1, pcols)[y0:y0 + pr_c, x0:x0 + pc_c]
     so = cref.settings(n, **s_opts)
-    _, ocount, _, mask, _ = cref.run(so, pc_c, slc3, amp3, val3, pr_c, size, size,
-                                     nthreads=min(16, os.cpu_count() or 1))
-    c64, p64 = fp64.run(np.ascontiguousarray(slc3, np.float32), ocount, mask,
-                        cref.settings_dict(so), pc_c, size, size)
-    out = {}
+    sd = cref.settings_dict(so)
+    ocoh, ocount, _, mask, _ = cref.run(so, pc_c, slc3, amp3, val3, pr_c, size, size,
+                                        nthreads=min(16, os.cpu_count() or 1))
+    s32 = np.ascontiguousarray(slc3, np.float32).reshape(-1, n, 2)
+    slc_c = s32[..., 0].astype(np.float64) + 1j * s32[..., 1].astype(np.float64)
+    c64, p64, near, _ = fp64.run(slc_c, ocount, mask, sd, pc_c, size, size, band=1e-5)
+    d32 = np.abs(ocoh - c64)
+    out = {"oracle_fp32": {"max_abs_dgamma": float(np.nanmax(d32)),
+                           "pixels_over_1e-4": int(np.sum(d32 > 1e-4))}}
     for form, (coh, count, pairs) in gpu_maps.items():
         cc = coh[y0:y0 + size, x0:x0 + size].reshape(-1)
         kc = count[y0:y0 + size, x0:x0 + size].reshape(-1)
         pc = pairs[:, y0:y0 + size, x0:x0 + size].reshape(pairs.shape[0], -1).T
         d = np.abs(cc - c64)
         dp = np.abs(pc - p64)
+        worst = int(np.nanargmax(d))
+        off = np.where(near == 0, d, np.nan)
         out[form] = {
             "max_abs_dgamma": float(np.nanmax(d)),
             "rms_dgamma": float(np.sqrt(np.nanmean(d ** 2))),
             "pixels_over_1e-4": int(np.sum(d > 1e-4)),
+            "max_abs_dgamma_off_bias_threshold": float(np.nanmax(off)),
             "max_abs_dpair": float(np.nanmax(dp)),
             "nan_mask_equal": bool(np.array_equal(np.isnan(cc), np.isnan(c64))),
             "shp_count_mismatches": int(np.sum(kc != ocount)),
+            "worst": dict(pixel_conditioning(fp64, slc_c, ocount, mask, sd, pc_c,
+                                             size, worst),
+                          gpu=float(cc[worst]), fp64=float(c64[worst]),
+                          oracle_fp32=float(ocoh[worst])),
         }
     return out
 

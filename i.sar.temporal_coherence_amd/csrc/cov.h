@@ -11,14 +11,14 @@
  *   sums in registers; per accepted candidate (dy-major, the reference's
  *   summation order) it loads the B row and B column samples it needs.
  *
- * mixed (cov_mfma_pixel): the whole wavefront on one pixel, as a real
- *   Gram matrix. With R the 2N x L matrix [Re s_k ; Im s_k] of the L
- *   accepted candidates, G = R R^T, C_re = G_rr + G_ii and
- *   C_im[u][v] = G_ri[v][u] - G_ri[u][v]. Each sample is split as
- *   hi + lo (FP16: x = hi + lo + O(2^-22 x); BF16: O(2^-18 x)) and G is
- *   accumulated in FP32 from hi.hi + hi.lo + lo.hi with
- *   v_mfma_f32_16x16x32_{f16,bf16}. The accepted candidates are compacted
- *   into an LDS list padded to a multiple of 32 with a zero pixel; A and B
+ * mixed (cov_mfma_band): the whole wavefront on one pixel at a time, as
+ *   real Gram products. With Re, Im the N x L matrices of the L accepted
+ *   candidates, C_re = Re Re^T + Im Im^T and C_im = Im Re^T - Re Im^T.
+ *   Each sample is split as hi + lo (FP16: x = hi + lo + O(2^-22 x); BF16:
+ *   O(2^-18 x)) and every product is accumulated in FP32 from hi.hi +
+ *   hi.lo + lo.hi with v_mfma_f32_16x16x32_{f16,bf16}, for the upper
+ *   16 x 16 date blocks only. The accepted candidates are compacted into
+ *   LDS lists padded to a multiple of 32 with a zero pixel; A and B
  *   operands share one lane layout, so the candidate order inside a
  *   32-candidate chunk does not matter. Dates are padded to NB with zeros.
  */
@@ -198,177 +198,310 @@ __device__ __forceinline__ Opnd make_opnd(const uint2 (&v)[8])
     return o;
 }
 
-/* LDS words the mixed form needs: candidate list, then the C_re and G_ri
-   tiles over it. */
-template <int NB> struct MfmaLds {
-    static constexpr int LIST = TCOH_MAX_WINDOW + 32;
-    static constexpr int TILES = 2 * NB * (NB + 1);
-    static constexpr int WORDS = LIST > TILES ? LIST : TILES;
-};
+/* Candidate lists of the mixed form: window bit numbers (dy * wr + dx) as
+   16-bit entries, padded to a multiple of 32 with LIST_PAD (the zero
+   pixel). */
+#define LIST_PAD 0xFFFFu
 
-/* Mixed form, whole wavefront on pixel p (uniform). hl[pix][date] =
-   (re, im) words of hi | lo << 16; pixel npad is zero. */
-template <int N, int NB, int BF16>
-__device__ __forceinline__ void cov_mfma_pixel(const KParams &kp,
-                                               const uint2 *__restrict__ hl,
-                                               const uint32_t *__restrict__ mask,
-                                               int p, int cnt, float *lds,
-                                               float2 *tl,
-                                               float *__restrict__ pairs)
+__host__ __device__ __forceinline__ int mfma_list_stride(int kw)
+{
+    return (kw + 31) & ~31;
+}
+
+/* LDS bytes of the mixed form for ppw pixels: lists, the diagonal of C,
+   the pixels' mask words. */
+__host__ __device__ __forceinline__ int mfma_scratch_bytes(int ppw, int kw,
+                                                           int words, int nb)
+{
+    return ppw * mfma_list_stride(kw) * 2 + nb * 4 + ppw * words * 4;
+}
+
+/* a[q] for a uniform q, without dynamic register indexing. */
+template <int P>
+__device__ __forceinline__ int pick(const int (&a)[P], int q)
+{
+    int v = a[0];
+
+#pragma unroll
+    for (int k = 1; k < P; k++)
+        v = q == k ? a[k] : v;
+    return v;
+}
+
+/* Mixed form over the group's pixels x0 .. x0+PPW-1 of output row y, the
+   whole wavefront on one pixel at a time. The mask words of all pixels are
+   staged once and every candidate list is built (mbcnt compaction) before
+   the first gather, so one gather stream, DEPTH chunks ahead of the
+   MFMAs, runs through all pixels without draining at pixel boundaries.
+   C_re = Re Re^T + Im Im^T and C_im = Im Re^T - Re Im^T accumulate in
+   registers for the upper 16 x 16 blocks only (the sign by flipping the
+   16-bit sign bits of Re); the epilogue normalises them from registers. */
+template <int N, int NB, int PPW, int BF16>
+__device__ __forceinline__ void cov_mfma_band(const KParams &kp,
+                                              const uint2 *__restrict__ hl,
+                                              const uint32_t *__restrict__ mask,
+                                              const int *__restrict__ count,
+                                              int y, int x0,
+                                              unsigned char *scratch,
+                                              float2 *tl,
+                                              float *__restrict__ pairs)
 {
     constexpr int NBK = NB / 16;             /* 16-date blocks */
     constexpr int NUP = NBK * (NBK + 1) / 2; /* upper block pairs */
     constexpr int NT = N * (N - 1) / 2;
-    int *list = reinterpret_cast<int *>(lds);
-    float(*gre)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(lds);
-    float(*gri)[NB + 1] = reinterpret_cast<float(*)[NB + 1]>(lds + NB * (NB + 1));
+    constexpr int DEPTH = NBK <= 2 ? 2 : 1;  /* chunks in flight */
     const int lane = threadIdx.x;
-    const int y = p / kp.cols, x = p - (p / kp.cols) * kp.cols;
+    const int kw = kp.wa * kp.wr, nw = kp.words;
+    const int lstride = mfma_list_stride(kw);
+    uint16_t *list = reinterpret_cast<uint16_t *>(scratch);
+    float *diag = reinterpret_cast<float *>(scratch + PPW * lstride * 2);
+    uint32_t *mw = reinterpret_cast<uint32_t *>(diag + NB);
     const size_t plane = (size_t)kp.rows * kp.cols;
+    const int p0 = y * kp.cols + x0;
+    const int npix = min(PPW, kp.cols - x0);
+    int cnt[PPW], nch[PPW];
 
-    if (cnt < kp.min_looks) {
-        if (kp.npairs)
-            for (int t = lane; t < kp.npairs; t += WAVE)
-                pairs[(size_t)t * plane + p] = __builtin_nanf("");
-        return;
-    }
-
-    /* Compact the accepted candidates of the window. */
-    const uint32_t *mk = mask + (size_t)p * kp.words;
-    const int kw = kp.wa * kp.wr;
-
+#pragma unroll
+    for (int q = 0; q < PPW; q++)
+        cnt[q] = q < npix ? count[p0 + q] : -1;
+    for (int t = lane; t < npix * nw; t += WAVE)
+        mw[t] = mask[(size_t)p0 * nw + t];
     wave_sync();
-    for (int b = lane; b < kw; b += WAVE) {
-        const int wd = b >> 5;
-        const uint32_t word = mk[wd];
 
-        if ((word >> (b & 31)) & 1u) {
-            int rank = __popc(word & ((1u << (b & 31)) - 1u));
+    int total = 0;
 
-            for (int t = 0; t < wd; t++)
-                rank += __popc(mk[t]);
-            const int dy = b / kp.wr, dx = b - dy * kp.wr;
-
-            list[rank] = (y + dy) * kp.w + x + dx;
+#pragma unroll
+    for (int q = 0; q < PPW; q++) {
+        nch[q] = 0;
+        if (cnt[q] < kp.min_looks) {
+            if (q < npix && kp.npairs)
+                for (int t = lane; t < kp.npairs; t += WAVE)
+                    pairs[(size_t)t * plane + p0 + q] = __builtin_nanf("");
+            continue;
         }
+        uint16_t *lq = list + q * lstride;
+        int base = 0;
+
+        for (int it = 0; it * 64 < kw; it++) {
+            unsigned long long m64 = mw[q * nw + 2 * it];
+
+            if (2 * it + 1 < nw)
+                m64 |= (unsigned long long)mw[q * nw + 2 * it + 1] << 32;
+            if (kw - it * 64 < 64)
+                m64 &= (1ull << (kw - it * 64)) - 1ull;
+            const unsigned rank = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(m64 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m64, 0u));
+
+            if ((m64 >> lane) & 1ull)
+                lq[base + rank] = (uint16_t)(it * 64 + lane);
+            base += __popcll(m64);
+        }
+        const int lpad = (base + 31) & ~31;
+
+        for (int t = base + lane; t < lpad; t += WAVE)
+            lq[t] = (uint16_t)LIST_PAD;
+        nch[q] = lpad >> 5;
+        total += nch[q];
     }
-    const int lpad = (cnt + 31) & ~31;
-    for (int t = cnt + lane; t < lpad; t += WAVE)
-        list[t] = kp.npad;
     wave_sync();
+    if (total == 0)
+        return;
 
     const int i16 = lane & 15, g = lane >> 4;
-    float4_t acc_rr[NUP], acc_ii[NUP], acc_ri[NBK * NBK];
+    const float inv_wr = 1.0f / (float)kp.wr;
+    float4_t cre[NUP], cim[NUP];
 
 #pragma unroll
     for (int t = 0; t < NUP; t++)
-        acc_rr[t] = acc_ii[t] = float4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int t = 0; t < NBK * NBK; t++)
-        acc_ri[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+        cre[t] = cim[t] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-    /* Gathers run two chunks ahead of the MFMAs. */
-    uint2 cur[NBK][8], nxt[NBK][8], nx2[NBK][8];
-    auto fetch = [&](int c0, uint2(&v)[NBK][8]) {
-        int idx[8];
+    /* Pixel and chunk of flat chunk c (pixels without estimate have no
+       chunks). */
+    auto locate = [&](int c, int &q, int &off) {
+        q = 0;
+        off = c;
+#pragma unroll
+        for (int k = 0; k < PPW - 1; k++)
+            if (q == k && off >= nch[k]) {
+                off -= nch[k];
+                q = k + 1;
+            }
+    };
+    auto fetch = [&](int c, uint2(&v)[NBK][8]) {
+        int q, off;
+
+        locate(c, q, off);
+        /* 8 entries of this lane's candidate group: one 16-byte LDS read
+           (list rows are 64-byte aligned). */
+        const uint4 raw = *reinterpret_cast<const uint4 *>(
+            list + q * lstride + off * 32 + g * 8);
+        const unsigned words[4] = {raw.x, raw.y, raw.z, raw.w};
+        const int origin = y * kp.w + x0 + q;
+        unsigned idx[8];
 
 #pragma unroll
-        for (int e = 0; e < 8; e++)
-            idx[e] = list[c0 + g * 8 + e];
+        for (int e = 0; e < 8; e++) {
+            const int b = (int)((words[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
+            /* b < 1024: (b + 0.5) / wr is at least 0.5 / wr away from an
+               integer, far more than the float product's error. */
+            const int dy = (int)(((float)b + 0.5f) * inv_wr);
+            const int dx = b - dy * kp.wr;
+
+            idx[e] = b == (int)LIST_PAD ? (unsigned)kp.npad
+                                        : (unsigned)(origin + dy * kp.w + dx);
+        }
 #pragma unroll
         for (int bk = 0; bk < NBK; bk++) {
             const int d = bk * 16 + i16;
 
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                v[bk][e] = d < N ? hl[(size_t)idx[e] * N + d] : make_uint2(0u, 0u);
+                v[bk][e] = d < N ? hl[idx[e] * (unsigned)N + d] : make_uint2(0u, 0u);
         }
     };
 
-    fetch(0, cur);
-    if (32 < lpad)
-        fetch(32, nxt);
-    for (int c0 = 0; c0 < lpad; c0 += 32) {
-        if (c0 + 64 < lpad)
-            fetch(c0 + 64, nx2);
+    uint2 buf[DEPTH + 1][NBK][8];
+
+#pragma unroll
+    for (int k = 0; k < DEPTH; k++)
+        if (k < total)
+            fetch(k, buf[k]);
+
+    int q, left;
+
+    locate(0, q, left);
+    left = pick(nch, q);
+    for (int c = 0; c < total; c++) {
+        if (c + DEPTH < total)
+            fetch(c + DEPTH, buf[DEPTH]);
         Opnd op[NBK];
+        uint4_t nrh[NBK], nrl[NBK];
 
 #pragma unroll
-        for (int bk = 0; bk < NBK; bk++)
-            op[bk] = make_opnd(cur[bk]);
-#pragma unroll
-        for (int bk = 0; bk < NBK; bk++)
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                cur[bk][e] = nxt[bk][e];
-                nxt[bk][e] = nx2[bk][e];
-            }
-        int t = 0;
-#pragma unroll
-        for (int a = 0; a < NBK; a++)
-#pragma unroll
-            for (int b = a; b < NBK; b++, t++) {
-                acc_rr[t] = mfma<BF16>(op[a].rh, op[b].rh, acc_rr[t]);
-                acc_rr[t] = mfma<BF16>(op[a].rh, op[b].rl, acc_rr[t]);
-                acc_rr[t] = mfma<BF16>(op[a].rl, op[b].rh, acc_rr[t]);
-                acc_ii[t] = mfma<BF16>(op[a].ih, op[b].ih, acc_ii[t]);
-                acc_ii[t] = mfma<BF16>(op[a].ih, op[b].il, acc_ii[t]);
-                acc_ii[t] = mfma<BF16>(op[a].il, op[b].ih, acc_ii[t]);
-            }
-#pragma unroll
-        for (int a = 0; a < NBK; a++)
-#pragma unroll
-            for (int b = 0; b < NBK; b++) {
-                float4_t &r = acc_ri[a * NBK + b];
-
-                r = mfma<BF16>(op[a].rh, op[b].ih, r);
-                r = mfma<BF16>(op[a].rh, op[b].il, r);
-                r = mfma<BF16>(op[a].rl, op[b].ih, r);
-            }
-    }
-
-    /* D[4g + r][i16] of each tile to LDS, over the dead list. */
-    wave_sync();
-    {
-        int t = 0;
-
-#pragma unroll
-        for (int a = 0; a < NBK; a++)
-#pragma unroll
-            for (int b = a; b < NBK; b++, t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    gre[a * 16 + 4 * g + r][b * 16 + i16] =
-                        acc_rr[t][r] + acc_ii[t][r];
-#pragma unroll
-        for (int a = 0; a < NBK; a++)
-#pragma unroll
-            for (int b = 0; b < NBK; b++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    gri[a * 16 + 4 * g + r][b * 16 + i16] =
-                        acc_ri[a * NBK + b][r];
-    }
-    wave_sync();
-
-    for (int e = lane; e < NT; e += WAVE) {
-        int i = 0, rem = e;
-
-        while (rem >= N - 1 - i) {
-            rem -= N - 1 - i;
-            i++;
+        for (int bk = 0; bk < NBK; bk++) {
+            op[bk] = make_opnd(buf[0][bk]);
+            nrh[bk] = op[bk].rh ^ 0x80008000u;
+            nrl[bk] = op[bk].rl ^ 0x80008000u;
         }
-        const int j = i + 1 + rem;
-        const float2 t = coherence_entry(gre[i][j], gri[j][i] - gri[i][j],
-                                         sqrtf(gre[i][i]) * sqrtf(gre[j][j]),
-                                         kp.bias, cnt);
-        const int pidx = pair_plane(kp, i, j);
+#pragma unroll
+        for (int k = 0; k < DEPTH; k++)
+#pragma unroll
+            for (int bk = 0; bk < NBK; bk++)
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    buf[k][bk][e] = buf[k + 1][bk][e];
 
-        tl[e] = t;
-        if (pidx >= 0)
-            pairs[(size_t)pidx * plane + p] = hypotf(t.x, t.y);
+        /* hi.hi + hi.lo + lo.hi of each product, the six accumulators of a
+           step independent of each other. */
+#define TCOH_UPPER(STMT)                                                       \
+    {                                                                          \
+        int t = 0;                                                             \
+        _Pragma("unroll") for (int a = 0; a < NBK; a++)                        \
+            _Pragma("unroll") for (int b = a; b < NBK; b++, t++) { STMT; }     \
     }
-    wave_sync();
+        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rh, cre[t]);
+                   cim[t] = mfma<BF16>(op[a].ih, op[b].rh, cim[t]))
+        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rl, cre[t]);
+                   cim[t] = mfma<BF16>(op[a].ih, op[b].rl, cim[t]))
+        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rl, op[b].rh, cre[t]);
+                   cim[t] = mfma<BF16>(op[a].il, op[b].rh, cim[t]))
+        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].ih, cre[t]);
+                   cim[t] = mfma<BF16>(nrh[a], op[b].ih, cim[t]))
+        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].il, cre[t]);
+                   cim[t] = mfma<BF16>(nrh[a], op[b].il, cim[t]))
+        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].il, op[b].ih, cre[t]);
+                   cim[t] = mfma<BF16>(nrl[a], op[b].ih, cim[t]))
+#undef TCOH_UPPER
+
+        if (--left > 0)
+            continue;
+
+        /* Last chunk of pixel q: T and the pair coherences from the
+           accumulators. Lane (i16, g) holds D[4g + r][i16] of each tile. */
+        const int p = p0 + q;
+        const int looks = pick(cnt, q);
+        float2 *tq = tl + q * NT;
+
+        {
+            int t = 0;
+
+#pragma unroll
+            for (int a = 0; a < NBK; a++)
+#pragma unroll
+                for (int b = a; b < NBK; b++, t++)
+                    if (a == b && (i16 >> 2) == g) {
+                        const int r = i16 & 3;
+                        const float v = r == 0 ? cre[t][0]
+                                        : r == 1 ? cre[t][1]
+                                        : r == 2 ? cre[t][2] : cre[t][3];
+
+                        diag[a * 16 + i16] = v;
+                    }
+        }
+        wave_sync();
+        float rsr[NBK][4], rsc[NBK];
+
+#pragma unroll
+        for (int a = 0; a < NBK; a++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float dv = diag[a * 16 + 4 * g + r];
+
+                rsr[a][r] = dv > 0.0f ? rsqrtf(dv) : 0.0f;
+            }
+            const float dv = diag[a * 16 + i16];
+
+            rsc[a] = dv > 0.0f ? rsqrtf(dv) : 0.0f;
+        }
+        const bool bias = kp.bias && looks > 1;
+        const float fl = (float)looks;
+        const float inv_l1 = bias ? 1.0f / (fl - 1.0f) : 0.0f;
+
+        {
+            int t = 0;
+
+#pragma unroll
+            for (int a = 0; a < NBK; a++)
+#pragma unroll
+                for (int b = a; b < NBK; b++, t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int i = a * 16 + 4 * g + r, j = b * 16 + i16;
+
+                        if (i >= j || j >= N)
+                            continue;
+                        const float sc = rsr[a][r] * rsc[b];
+                        float re = cre[t][r] * sc, im = cim[t][r] * sc;
+
+                        if (bias) {
+                            /* tcoh.cl:620-633 */
+                            const float m2 = re * re + im * im;
+                            const float m2c = (fl * m2 - 1.0f) * inv_l1;
+                            const float f = (m2 > 0.0f && m2c > 0.0f)
+                                                ? sqrtf(m2c) * rsqrtf(m2)
+                                                : 0.0f;
+
+                            re *= f;
+                            im *= f;
+                        }
+                        tq[upper_index(N, i, j)] = make_float2(re, im);
+                        const int pidx = pair_plane(kp, i, j);
+
+                        if (pidx >= 0)
+                            pairs[(size_t)pidx * plane + p] = sqrtf(re * re + im * im);
+                    }
+        }
+#pragma unroll
+        for (int t = 0; t < NUP; t++)
+            cre[t] = cim[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+        if (c + 1 < total) {
+            int off;
+
+            locate(c + 1, q, off);
+            left = pick(nch, q) - off;
+        }
+        wave_sync();
+    }
 }
 
 #endif

@@ -40,20 +40,26 @@
 #define MAX_SWEEPS 40
 
 /* vw[r] = (v_r, w_r) of the current Householder step, read as one
-   16-byte word by the update. */
+   16-byte word by the update. vw is dead once the reduction ends, and the
+   inverse iteration's LU factors (u0 holds 1/pivot) and result y take its
+   place. */
 template <int N> struct EigLds {
-    float4 vw[N];
+    union {
+        float4 vw[N];
+        struct {
+            float u0[N], u1[N], u2[N], l[N], y[N];
+            int piv[N];
+        };
+    };
     float2 tau[N];
     float d[N], e[N];
-    float u0[N], u1[N], u2[N], l[N], y[N];
-    int piv[N];
     float2 z[N];
     float2 alpha;
 };
 
 /* Cholesky factor (FP64, lower, column by column) or, in the fallback,
    the Jacobi matrix and its eigenvectors (FP32). */
-template <int N> union EmiLds {
+template <int N> union __attribute__((aligned(16))) EmiLds {
     double lc[N * (N + 1) / 2 + 1];
     struct {
         float a[N * N];
@@ -315,13 +321,13 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
         }
         sh.l[k] = lk;
         sh.piv[k] = pv;
-        sh.u0[k] = a0;
+        sh.u0[k] = 1.0f / a0;
         sh.u1[k] = a1;
         sh.u2[k] = a2;
     }
     if (fabsf(dd) < tiny)
         dd = dd < 0.0f ? -tiny : tiny;
-    sh.u0[N - 1] = dd;
+    sh.u0[N - 1] = 1.0f / dd;
     sh.u1[N - 1] = 0.0f;
     sh.u2[N - 1] = 0.0f;
 
@@ -349,14 +355,16 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
                 v -= sh.u1[k] * y[k + 1];
             if (k + 2 < N)
                 v -= sh.u2[k] * y[k + 2];
-            y[k] = v / sh.u0[k];
+            y[k] = v * sh.u0[k];
         }
 #pragma unroll
         for (int k = 0; k < N; k++)
             nrm = fmaxf(nrm, fabsf(y[k]));
+        const float rn = 1.0f / nrm;
+
 #pragma unroll
         for (int k = 0; k < N; k++)
-            y[k] /= nrm;
+            y[k] *= rn;
     }
     if (lg == 0) {
 #pragma unroll

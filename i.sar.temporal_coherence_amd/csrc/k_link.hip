@@ -2,9 +2,11 @@
  * 567-744): each work-group (one wavefront) takes PPW = 64/G horizontally
  * adjacent pixels, builds their coherence matrices T in LDS (cov.h, plain
  * or mixed precision) and links them with G lanes per pixel (link.h), so T
- * never leaves the compute unit. Groups are tiled XCD by XCD over column
- * stripes (common.h) so that the windows of one stripe stay in that XCD's
- * L2. Instantiated once per stack depth N (one translation unit each).
+ * never leaves the compute unit. The LDS is sized per launch from the
+ * window (a 21x7 window at N=30 takes under 20 KB per group). Groups are
+ * tiled XCD by XCD over column stripes (common.h) so that the windows of
+ * one stripe stay in that XCD's L2. Instantiated once per stack depth N
+ * (one translation unit each).
  */
 
 #include "cov.h"
@@ -15,28 +17,41 @@ template <int N> struct LinkShape {
     static constexpr int PPW = WAVE / G;
     static constexpr int NT = N * (N - 1) / 2;
     static constexpr int NB = N <= 16 ? 16 : (N <= 32 ? 32 : 64);
+    /* Dynamic LDS of a group: T of each pixel (alive through both stages),
+       then a scratch area the covariance and the linking take in turn,
+       then (EMI) the inversion areas. */
+    static constexpr int TL_BYTES = (PPW * NT * 8 + 15) & ~15;
+    static constexpr int EIG_BYTES = PPW * (int)sizeof(EigLds<N>);
+    static constexpr int EMI_BYTES = PPW * (int)sizeof(EmiLds<N>);
 };
+
+template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
+{
+    typedef LinkShape<N> S;
+    const int cov = precision ? mfma_scratch_bytes(S::PPW, kp.wa * kp.wr, kp.words, S::NB)
+                              : S::PPW * N * 4;
+
+    return ((cov > S::EIG_BYTES ? cov : S::EIG_BYTES) + 15) & ~15;
+}
 
 /* COV: 0 plain FP32, 1 split FP16 MFMA, 2 split BF16 MFMA. */
 template <int N, int COV, int EMI>
-__global__ __launch_bounds__(64, 2) void k_link(KParams kp,
+__global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
                                                const uint2 *__restrict__ hl,
                                                const uint32_t *__restrict__ mask,
                                                const int *__restrict__ count,
                                                float *__restrict__ coh,
                                                float *__restrict__ pairs,
-                                               int row0, int nrows)
+                                               int row0, int nrows,
+                                               int scratch_bytes)
 {
     typedef LinkShape<N> S;
     constexpr int G = S::G, PPW = S::PPW, NT = S::NT, NB = S::NB;
-    constexpr int WC = COV ? MfmaLds<NB>::WORDS : PPW * N;
-    constexpr int WE = (int)(PPW * sizeof(EigLds<N>) / 4);
-    /* T of each group's pixel, alive through both stages; the covariance
-       scratch and the linking scratch share the rest. */
-    __shared__ float2 tl[PPW][NT];
-    __shared__ __attribute__((aligned(16))) float scratch[WC > WE ? WC : WE];
-    __shared__ EmiLds<N> ems[EMI ? PPW : 1];
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    float2 *tl = reinterpret_cast<float2 *>(lds);
+    unsigned char *scratch = lds + S::TL_BYTES;
+    EmiLds<N> *ems = reinterpret_cast<EmiLds<N> *>(scratch + scratch_bytes);
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
     int r, tx;
 
@@ -48,25 +63,16 @@ __global__ __launch_bounds__(64, 2) void k_link(KParams kp,
     const int cnt = live ? count[p] : -1;
     const bool est = live && cnt >= kp.min_looks;
 
-    if (COV == 0) {
+    if (COV == 0)
         cov_plain_group<N, G>(kp, slc, mask, p, cnt, est, live, lg,
-                              scratch + gi * N, tl[gi], pairs);
-    }
-    else {
-        for (int g = 0; g < PPW; g++) {
-            const int xg = tx * PPW + g;
-
-            if (xg >= kp.cols)
-                break;
-            const int pg = (row0 + r) * kp.cols + xg;
-
-            cov_mfma_pixel<N, NB, COV == 2>(kp, hl, mask, pg, count[pg], scratch,
-                                            tl[g], pairs);
-        }
-    }
+                              reinterpret_cast<float *>(scratch) + gi * N,
+                              tl + gi * NT, pairs);
+    else
+        cov_mfma_band<N, NB, PPW, COV == 2>(kp, hl, mask, count, row0 + r,
+                                            tx * PPW, scratch, tl, pairs);
     wave_sync();
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(scratch);
-    const float gam = link_group<N, G, EMI>(kp, tl[gi], est, lg, sh[gi],
+    const float gam = link_group<N, G, EMI>(kp, tl + gi * NT, est, lg, sh[gi],
                                             ems[EMI ? gi : 0]);
 
     if (live && lg == 0)
@@ -81,10 +87,22 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
 {
     typedef LinkShape<N> S;
     const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW));
+    const int scratch = link_scratch_bytes<N>(kp, precision);
+    const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::EMI_BYTES : 0);
 
-#define TCOH_LINK_LAUNCH(COV, EMI)                                           \
-    hipLaunchKernelGGL((k_link<N, COV, EMI>), grid, dim3(WAVE), 0, s, kp,    \
-                       slc, hl, mask, count, coh, pairs, row0, nrows)
+#define TCOH_LINK_LAUNCH(COV, EMI)                                              \
+    do {                                                                        \
+        if (lds > 65536) {                                                      \
+            const hipError_t e = hipFuncSetAttribute(                           \
+                reinterpret_cast<const void *>(&k_link<N, COV, EMI>),           \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);          \
+            if (e != hipSuccess)                                                \
+                return e;                                                       \
+        }                                                                       \
+        hipLaunchKernelGGL((k_link<N, COV, EMI>), grid, dim3(WAVE), lds, s, kp, \
+                           slc, hl, mask, count, coh, pairs, row0, nrows,       \
+                           scratch);                                            \
+    } while (0)
     if (kp.emi) {
         if (precision == 0)
             TCOH_LINK_LAUNCH(0, 1);
