@@ -427,7 +427,7 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
         const int bf16 = o->precision == TCOH_MIXED_BF16;
         float scale = 1.0f;
 
-        o->hl.ensure((npad + 1) * n * 2 * sizeof(uint32_t), "hipMalloc(hl)");
+        o->hl.ensure((npad + 1) * shp_bucket(n) * 2 * sizeof(uint32_t), "hipMalloc(hl)");
         if (!bf16) {
             unsigned int bits = 0;
 

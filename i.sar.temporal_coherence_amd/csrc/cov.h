@@ -209,23 +209,11 @@ __host__ __device__ __forceinline__ int mfma_list_stride(int kw)
 }
 
 /* LDS bytes of the mixed form for ppw pixels: lists, the diagonal of C,
-   the pixels' mask words. */
+   the pixels' mask words, and a per-pixel table (chunks, looks). */
 __host__ __device__ __forceinline__ int mfma_scratch_bytes(int ppw, int kw,
                                                            int words, int nb)
 {
-    return ppw * mfma_list_stride(kw) * 2 + nb * 4 + ppw * words * 4;
-}
-
-/* a[q] for a uniform q, without dynamic register indexing. */
-template <int P>
-__device__ __forceinline__ int pick(const int (&a)[P], int q)
-{
-    int v = a[0];
-
-#pragma unroll
-    for (int k = 1; k < P; k++)
-        v = q == k ? a[k] : v;
-    return v;
+    return ppw * mfma_list_stride(kw) * 2 + nb * 4 + ppw * words * 4 + ppw * 8;
 }
 
 /* Mixed form over the group's pixels x0 .. x0+PPW-1 of output row y, the
@@ -256,6 +244,11 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     uint16_t *list = reinterpret_cast<uint16_t *>(scratch);
     float *diag = reinterpret_cast<float *>(scratch + PPW * lstride * 2);
     uint32_t *mw = reinterpret_cast<uint32_t *>(diag + NB);
+    /* chunks and looks of pixel q at ptab[q], ptab[PPW + q]: read back with
+       a uniform address when q is only known at run time (a register
+       array indexed so would become a scratch-memory table, whose loads
+       would wait behind the gathers in flight) */
+    int *ptab = reinterpret_cast<int *>(mw + PPW * nw);
     const size_t plane = (size_t)kp.rows * kp.cols;
     const int p0 = y * kp.cols + x0;
     const int npix = min(PPW, kp.cols - x0);
@@ -263,7 +256,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 
 #pragma unroll
     for (int q = 0; q < PPW; q++)
-        cnt[q] = q < npix ? count[p0 + q] : -1;
+        cnt[q] = q < npix ? __builtin_amdgcn_readfirstlane(count[p0 + q]) : -1;
     for (int t = lane; t < npix * nw; t += WAVE)
         mw[t] = mask[(size_t)p0 * nw + t];
     wave_sync();
@@ -283,10 +276,15 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         int base = 0;
 
         for (int it = 0; it * 64 < kw; it++) {
-            unsigned long long m64 = mw[q * nw + 2 * it];
+            /* The words are uniform: read them into scalar registers, so
+               that the counts and every branch on them stay uniform. */
+            unsigned long long m64 =
+                (unsigned)__builtin_amdgcn_readfirstlane((int)mw[q * nw + 2 * it]);
 
             if (2 * it + 1 < nw)
-                m64 |= (unsigned long long)mw[q * nw + 2 * it + 1] << 32;
+                m64 |= (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
+                           (int)mw[q * nw + 2 * it + 1])
+                       << 32;
             if (kw - it * 64 < 64)
                 m64 &= (1ull << (kw - it * 64)) - 1ull;
             const unsigned rank = __builtin_amdgcn_mbcnt_hi(
@@ -302,6 +300,13 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
             lq[t] = (uint16_t)LIST_PAD;
         nch[q] = lpad >> 5;
         total += nch[q];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < PPW; q++) {
+            ptab[q] = nch[q];
+            ptab[PPW + q] = cnt[q];
+        }
     }
     wave_sync();
     if (total == 0)
@@ -327,17 +332,18 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                 q = k + 1;
             }
     };
+    /* Samples of flat chunk c for this lane: dates bk*16 + i16 of the 8
+       candidates of its group g. hl rows hold NB date slots (zero past N),
+       so every load is unconditional. */
     auto fetch = [&](int c, uint2(&v)[NBK][8]) {
         int q, off;
 
         locate(c, q, off);
-        /* 8 entries of this lane's candidate group: one 16-byte LDS read
-           (list rows are 64-byte aligned). */
+        /* one 16-byte LDS read (list rows are 64-byte aligned) */
         const uint4 raw = *reinterpret_cast<const uint4 *>(
             list + q * lstride + off * 32 + g * 8);
         const unsigned words[4] = {raw.x, raw.y, raw.z, raw.w};
         const int origin = y * kp.w + x0 + q;
-        unsigned idx[8];
 
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -346,52 +352,35 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                integer, far more than the float product's error. */
             const int dy = (int)(((float)b + 0.5f) * inv_wr);
             const int dx = b - dy * kp.wr;
-
-            idx[e] = b == (int)LIST_PAD ? (unsigned)kp.npad
-                                        : (unsigned)(origin + dy * kp.w + dx);
-        }
-#pragma unroll
-        for (int bk = 0; bk < NBK; bk++) {
-            const int d = bk * 16 + i16;
+            const unsigned idx = b == (int)LIST_PAD
+                                     ? (unsigned)kp.npad
+                                     : (unsigned)(origin + dy * kp.w + dx);
+            const uint2 *row = hl + idx * (unsigned)NB + i16;
 
 #pragma unroll
-            for (int e = 0; e < 8; e++)
-                v[bk][e] = d < N ? hl[idx[e] * (unsigned)N + d] : make_uint2(0u, 0u);
+            for (int bk = 0; bk < NBK; bk++)
+                v[bk][e] = row[bk * 16];
         }
     };
-
-    uint2 buf[DEPTH + 1][NBK][8];
-
-#pragma unroll
-    for (int k = 0; k < DEPTH; k++)
-        if (k < total)
-            fetch(k, buf[k]);
 
     int q, left;
 
     locate(0, q, left);
-    left = pick(nch, q);
-    for (int c = 0; c < total; c++) {
-        if (c + DEPTH < total)
-            fetch(c + DEPTH, buf[DEPTH]);
+    left = __builtin_amdgcn_readfirstlane(ptab[q]);
+
+    /* One chunk: MFMAs on use (chunk c), and at the pixel's last chunk the
+       epilogue. */
+    auto step = [&](int c, const uint2(&use)[NBK][8]) {
         Opnd op[NBK];
         uint4_t nrh[NBK], nrl[NBK];
 
 #pragma unroll
         for (int bk = 0; bk < NBK; bk++) {
-            op[bk] = make_opnd(buf[0][bk]);
+            op[bk] = make_opnd(use[bk]);
             nrh[bk] = op[bk].rh ^ 0x80008000u;
             nrl[bk] = op[bk].rl ^ 0x80008000u;
         }
-#pragma unroll
-        for (int k = 0; k < DEPTH; k++)
-#pragma unroll
-            for (int bk = 0; bk < NBK; bk++)
-#pragma unroll
-                for (int e = 0; e < 8; e++)
-                    buf[k][bk][e] = buf[k + 1][bk][e];
-
-        /* hi.hi + hi.lo + lo.hi of each product, the six accumulators of a
+        /* hi.hi + hi.lo + lo.hi of each product, the accumulators of a
            step independent of each other. */
 #define TCOH_UPPER(STMT)                                                       \
     {                                                                          \
@@ -414,12 +403,11 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 #undef TCOH_UPPER
 
         if (--left > 0)
-            continue;
+            return;
 
-        /* Last chunk of pixel q: T and the pair coherences from the
-           accumulators. Lane (i16, g) holds D[4g + r][i16] of each tile. */
-        const int p = p0 + q;
-        const int looks = pick(cnt, q);
+        /* Last chunk of pixel q: T from the accumulators, lane (i16, g)
+           holding D[4g + r][i16] of each tile. */
+        const int looks = __builtin_amdgcn_readfirstlane(ptab[PPW + q]);
         float2 *tq = tl + q * NT;
 
         {
@@ -456,51 +444,90 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         const bool bias = kp.bias && looks > 1;
         const float fl = (float)looks;
         const float inv_l1 = bias ? 1.0f / (fl - 1.0f) : 0.0f;
-
-        {
-            int t = 0;
+        int t = 0;
 
 #pragma unroll
-            for (int a = 0; a < NBK; a++)
+        for (int a = 0; a < NBK; a++)
 #pragma unroll
-                for (int b = a; b < NBK; b++, t++)
+            for (int b = a; b < NBK; b++, t++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int i = a * 16 + 4 * g + r, j = b * 16 + i16;
+                for (int r = 0; r < 4; r++) {
+                    const int i = a * 16 + 4 * g + r, j = b * 16 + i16;
+                    const float sc = rsr[a][r] * rsc[b];
+                    float re = cre[t][r] * sc, im = cim[t][r] * sc;
 
-                        if (i >= j || j >= N)
-                            continue;
-                        const float sc = rsr[a][r] * rsc[b];
-                        float re = cre[t][r] * sc, im = cim[t][r] * sc;
+                    if (bias) {
+                        /* tcoh.cl:620-633 */
+                        const float m2 = re * re + im * im;
+                        const float m2c = (fl * m2 - 1.0f) * inv_l1;
+                        const float f = (m2 > 0.0f && m2c > 0.0f)
+                                            ? sqrtf(m2c) * rsqrtf(m2)
+                                            : 0.0f;
 
-                        if (bias) {
-                            /* tcoh.cl:620-633 */
-                            const float m2 = re * re + im * im;
-                            const float m2c = (fl * m2 - 1.0f) * inv_l1;
-                            const float f = (m2 > 0.0f && m2c > 0.0f)
-                                                ? sqrtf(m2c) * rsqrtf(m2)
-                                                : 0.0f;
-
-                            re *= f;
-                            im *= f;
-                        }
-                        tq[upper_index(N, i, j)] = make_float2(re, im);
-                        const int pidx = pair_plane(kp, i, j);
-
-                        if (pidx >= 0)
-                            pairs[(size_t)pidx * plane + p] = sqrtf(re * re + im * im);
+                        re *= f;
+                        im *= f;
                     }
-        }
+                    if (i < j && j < N)
+                        tq[upper_index(N, i, j)] = make_float2(re, im);
+                }
 #pragma unroll
-        for (int t = 0; t < NUP; t++)
-            cre[t] = cim[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < NUP; u++)
+            cre[u] = cim[u] = float4_t{0.f, 0.f, 0.f, 0.f};
         if (c + 1 < total) {
             int off;
 
             locate(c + 1, q, off);
-            left = pick(nch, q) - off;
+            left = __builtin_amdgcn_readfirstlane(ptab[q]) - off;
         }
         wave_sync();
+    };
+
+    /* Buffers named statically (no register copies) and fetched on every
+       path (past the end the last chunk again), so that each step waits
+       for the loads of its own chunk only. */
+    uint2 b0[NBK][8], b1[NBK][8], b2[NBK][8];
+    const int last = total - 1;
+
+    if (DEPTH == 2) {
+        fetch(0, b0);
+        fetch(min(1, last), b1);
+        for (int c = 0; c < total; c += 3) {
+            fetch(min(c + 2, last), b2);
+            step(c, b0);
+            if (c + 1 >= total)
+                break;
+            fetch(min(c + 3, last), b0);
+            step(c + 1, b1);
+            if (c + 2 >= total)
+                break;
+            fetch(min(c + 4, last), b1);
+            step(c + 2, b2);
+        }
+    }
+    else {
+        fetch(0, b0);
+        for (int c = 0; c < total; c += 2) {
+            fetch(min(c + 1, last), b1);
+            step(c, b0);
+            if (c + 1 >= total)
+                break;
+            fetch(min(c + 2, last), b0);
+            step(c + 1, b1);
+        }
+    }
+
+    /* Pair coherences |T_ij| of the estimated pixels, from LDS. */
+    if (kp.npairs) {
+#pragma unroll
+        for (int qq = 0; qq < PPW; qq++) {
+            if (nch[qq] == 0)
+                continue;
+            for (int t = lane; t < kp.npairs; t += WAVE) {
+                const float2 v = tl[qq * NT + (kp.pairs_all ? t : upper_index(N, t, t + 1))];
+
+                pairs[(size_t)t * plane + p0 + qq] = sqrtf(v.x * v.x + v.y * v.y);
+            }
+        }
     }
 }
 

@@ -1,8 +1,10 @@
 /* Preparation of the mixed-precision covariance: every sample component x
  * (after a common power-of-two scale of the strip) becomes a 32-bit word
  * hi | lo << 16 with hi = round16(x), lo = round16(x - hi), FP16 or BF16,
- * round to nearest even. One zero pixel is appended (index npad) for the
- * padding of candidate lists. */
+ * round to nearest even. Each pixel takes NB = shp_bucket(N) date slots
+ * (slots N .. NB-1 zero), so that the covariance gathers whole, aligned
+ * 16-date blocks without guards; one zero pixel is appended (index npad)
+ * for the padding of candidate lists. */
 
 #include "common.h"
 
@@ -42,12 +44,16 @@ __global__ __launch_bounds__(256) void k_split(KParams kp,
                                               uint2 *__restrict__ hl, int bf16,
                                               float scale)
 {
+    const int nb = shp_bucket(kp.n);
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t total = ((size_t)kp.npad + 1) * kp.n;
+    const size_t total = ((size_t)kp.npad + 1) * nb;
 
     if (i >= total)
         return;
-    const float2 v = i < (size_t)kp.npad * kp.n ? slc[i] : make_float2(0.f, 0.f);
+    const size_t pix = i / nb;
+    const int d = (int)(i - pix * nb);
+    const float2 v = (pix < (size_t)kp.npad && d < kp.n) ? slc[pix * kp.n + d]
+                                                          : make_float2(0.f, 0.f);
     const float c[2] = {v.x * scale, v.y * scale};
     uint32_t w[2];
 
@@ -96,7 +102,7 @@ __global__ void k_zero_u32(unsigned int *out) { *out = 0u; }
 hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
                         int bf16, float scale, hipStream_t s)
 {
-    const size_t total = ((size_t)kp.npad + 1) * kp.n;
+    const size_t total = ((size_t)kp.npad + 1) * shp_bucket(kp.n);
 
     hipLaunchKernelGGL(k_split, dim3((unsigned)((total + 255) / 256)), dim3(256),
                        0, s, kp, slc, (uint2 *)hl, bf16, scale);
