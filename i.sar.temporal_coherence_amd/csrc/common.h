@@ -57,6 +57,14 @@ __host__ __device__ __forceinline__ int shp_bucket(int n)
     return n <= 16 ? 16 : (n <= 32 ? 32 : 64);
 }
 
+/* Hardware reciprocal (1 ulp). Used where a quotient only feeds a
+   result checked to a tolerance, never in a decision that must match the
+   reference bit for bit. */
+__device__ __forceinline__ float rcpf(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

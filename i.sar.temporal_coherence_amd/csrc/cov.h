@@ -198,11 +198,9 @@ __device__ __forceinline__ Opnd make_opnd(const uint2 (&v)[8])
     return o;
 }
 
-/* Candidate lists of the mixed form: window bit numbers (dy * wr + dx) as
-   16-bit entries, padded to a multiple of 32 with LIST_PAD (the zero
-   pixel). */
-#define LIST_PAD 0xFFFFu
-
+/* Candidate lists of the mixed form: the strip pixel index of every
+   accepted candidate, padded to a multiple of 32 with the zero pixel
+   (index npad). */
 __host__ __device__ __forceinline__ int mfma_list_stride(int kw)
 {
     return (kw + 31) & ~31;
@@ -213,7 +211,7 @@ __host__ __device__ __forceinline__ int mfma_list_stride(int kw)
 __host__ __device__ __forceinline__ int mfma_scratch_bytes(int ppw, int kw,
                                                            int words, int nb)
 {
-    return ppw * mfma_list_stride(kw) * 2 + nb * 4 + ppw * words * 4 + ppw * 8;
+    return ppw * mfma_list_stride(kw) * 4 + nb * 4 + ppw * words * 4 + ppw * 8;
 }
 
 /* Mixed form over the group's pixels x0 .. x0+PPW-1 of output row y, the
@@ -241,8 +239,8 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     const int lane = threadIdx.x;
     const int kw = kp.wa * kp.wr, nw = kp.words;
     const int lstride = mfma_list_stride(kw);
-    uint16_t *list = reinterpret_cast<uint16_t *>(scratch);
-    float *diag = reinterpret_cast<float *>(scratch + PPW * lstride * 2);
+    int *list = reinterpret_cast<int *>(scratch);
+    float *diag = reinterpret_cast<float *>(scratch + PPW * lstride * 4);
     uint32_t *mw = reinterpret_cast<uint32_t *>(diag + NB);
     /* chunks and looks of pixel q at ptab[q], ptab[PPW + q]: read back with
        a uniform address when q is only known at run time (a register
@@ -261,6 +259,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         mw[t] = mask[(size_t)p0 * nw + t];
     wave_sync();
 
+    const float inv_wr = 1.0f / (float)kp.wr;
     int total = 0;
 
 #pragma unroll
@@ -272,7 +271,8 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                     pairs[(size_t)t * plane + p0 + q] = __builtin_nanf("");
             continue;
         }
-        uint16_t *lq = list + q * lstride;
+        int *lq = list + q * lstride;
+        const int origin = y * kp.w + x0 + q;
         int base = 0;
 
         for (int it = 0; it * 64 < kw; it++) {
@@ -290,14 +290,20 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
             const unsigned rank = __builtin_amdgcn_mbcnt_hi(
                 (unsigned)(m64 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m64, 0u));
 
-            if ((m64 >> lane) & 1ull)
-                lq[base + rank] = (uint16_t)(it * 64 + lane);
+            if ((m64 >> lane) & 1ull) {
+                /* b < 1024: (b + 0.5) / wr is at least 0.5 / wr away from
+                   an integer, far more than the float product's error. */
+                const int b = it * 64 + lane;
+                const int dy = (int)(((float)b + 0.5f) * inv_wr);
+
+                lq[base + rank] = origin + dy * kp.w + (b - dy * kp.wr);
+            }
             base += __popcll(m64);
         }
         const int lpad = (base + 31) & ~31;
 
         for (int t = base + lane; t < lpad; t += WAVE)
-            lq[t] = (uint16_t)LIST_PAD;
+            lq[t] = kp.npad;
         nch[q] = lpad >> 5;
         total += nch[q];
     }
@@ -313,7 +319,6 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         return;
 
     const int i16 = lane & 15, g = lane >> 4;
-    const float inv_wr = 1.0f / (float)kp.wr;
     float4_t cre[NUP], cim[NUP];
 
 #pragma unroll
@@ -339,23 +344,14 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         int q, off;
 
         locate(c, q, off);
-        /* one 16-byte LDS read (list rows are 64-byte aligned) */
-        const uint4 raw = *reinterpret_cast<const uint4 *>(
-            list + q * lstride + off * 32 + g * 8);
-        const unsigned words[4] = {raw.x, raw.y, raw.z, raw.w};
-        const int origin = y * kp.w + x0 + q;
+        /* two 16-byte LDS reads (list rows are 128-byte aligned) */
+        const int4 *lp = reinterpret_cast<const int4 *>(list + q * lstride + off * 32 + g * 8);
+        const int4 i0 = lp[0], i1 = lp[1];
+        const int idx[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
 
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const int b = (int)((words[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
-            /* b < 1024: (b + 0.5) / wr is at least 0.5 / wr away from an
-               integer, far more than the float product's error. */
-            const int dy = (int)(((float)b + 0.5f) * inv_wr);
-            const int dx = b - dy * kp.wr;
-            const unsigned idx = b == (int)LIST_PAD
-                                     ? (unsigned)kp.npad
-                                     : (unsigned)(origin + dy * kp.w + dx);
-            const uint2 *row = hl + idx * (unsigned)NB + i16;
+            const uint2 *row = hl + (size_t)(unsigned)idx[e] * NB + i16;
 
 #pragma unroll
             for (int bk = 0; bk < NBK; bk++)

@@ -114,14 +114,16 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             beta = alr;
         else {
             beta = -copysignf(sqrtf(alr * alr + ali * ali + xn), alr);
-            tr_ = (beta - alr) / beta;
-            ti_ = -ali / beta;
+            const float rb = rcpf(beta);
+
+            tr_ = (beta - alr) * rb;
+            ti_ = -ali * rb;
             {
                 const float dr = alr - beta;
-                const float den = dr * dr + ali * ali;
+                const float rd = rcpf(dr * dr + ali * ali);
 
-                sr = dr / den;
-                si = -ali / den;
+                sr = dr * rd;
+                si = -ali * rd;
             }
         }
         float2 vc[CPL];
@@ -302,7 +304,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
         if (fabsf(dd) >= fabsf(sub)) {
             if (fabsf(dd) < tiny)
                 dd = dd < 0.0f ? -tiny : tiny;
-            lk = sub / dd;
+            lk = sub * rcpf(dd);
             pv = 0;
             a0 = dd;
             a1 = s1;
@@ -311,7 +313,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
             s1 = ns;
         }
         else {
-            lk = dd / sub;
+            lk = dd * rcpf(sub);
             pv = 1;
             a0 = sub;
             a1 = nd;
@@ -321,13 +323,13 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
         }
         sh.l[k] = lk;
         sh.piv[k] = pv;
-        sh.u0[k] = 1.0f / a0;
+        sh.u0[k] = rcpf(a0);
         sh.u1[k] = a1;
         sh.u2[k] = a2;
     }
     if (fabsf(dd) < tiny)
         dd = dd < 0.0f ? -tiny : tiny;
-    sh.u0[N - 1] = 1.0f / dd;
+    sh.u0[N - 1] = rcpf(dd);
     sh.u1[N - 1] = 0.0f;
     sh.u2[N - 1] = 0.0f;
 
@@ -360,7 +362,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
 #pragma unroll
         for (int k = 0; k < N; k++)
             nrm = fmaxf(nrm, fabsf(y[k]));
-        const float rn = 1.0f / nrm;
+        const float rn = rcpf(nrm);
 
 #pragma unroll
         for (int k = 0; k < N; k++)

@@ -102,7 +102,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     for (int m = 0; m < CPL; m++) {
         const int c = lg + m * G;
 
-        mags[m] = hypotf(z[m].x, z[m].y);
+        mags[m] = sqrtf(fmaf(z[m].x, z[m].x, z[m].y * z[m].y));
         if (c < N) {
             lmax = fmaxf(lmax, mags[m]);
             sh.z[c] = z[m];
@@ -114,7 +114,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     {
         const float2 zr = sh.z[N / 2];
 
-        if (hypotf(zr.x, zr.y) <= PHASOR_FLOOR * lmax) {
+        if (sqrtf(fmaf(zr.x, zr.x, zr.y * zr.y)) <= PHASOR_FLOOR * lmax) {
             int last = -1;
 
 #pragma unroll
@@ -131,7 +131,8 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     }
     const float2 zref = sh.z[ref];
     const float refm = hypotf(zref.x, zref.y);
-    const float refr = zref.x / refm, refi = -zref.y / refm;
+    const float rrm = rcpf(refm);
+    const float refr = zref.x * rrm, refi = -zref.y * rrm;
     const float floor_ = lmax * PHASOR_FLOOR;
     wave_sync();
 #pragma unroll
@@ -139,9 +140,10 @@ __device__ __forceinline__ float link_group(const KParams &kp,
         const int c = lg + m * G;
         const float re = z[m].x * refr - z[m].y * refi;
         const float im = z[m].x * refi + z[m].y * refr;
-        const float mg = hypotf(re, im);
+        const float mg = sqrtf(fmaf(re, re, im * im));
+        const float rmg = rcpf(mg);
 
-        z[m] = mg > floor_ ? make_float2(re / mg, im / mg) : make_float2(1.f, 0.f);
+        z[m] = mg > floor_ ? make_float2(re * rmg, im * rmg) : make_float2(1.f, 0.f);
         if (c < N)
             sh.z[c] = z[m];
     }
