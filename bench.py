@@ -31,6 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s at N=30, 21x7 KS+EVD: mixed-prec vs plain FP32 GPU, |dγ|<=1e-4"
+METRIC_EMI = "Mpixel/s at N=30, 21x7 AD+EMI, all pairs: mixed-prec vs plain FP32 GPU"
+PEAK_FP64 = 78.6e12    # MI355X FP64 vector (TFLOP/s)
 PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
 PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
 
@@ -165,6 +167,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=384)
     ap.add_argument("--crop", type=int, default=256)
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
+                    help="2: KS+EVD, bias, consecutive pairs, gamma0 ~ U(0.05, 0.95) "
+                         "(the headline); 3: AD+EMI, bias, all 435 pairs, "
+                         "gamma0 ~ U(0.6, 0.99)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device three times: a kernel of "
+                         "known traffic to calibrate HBM counters against")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -180,12 +189,31 @@ def main():
     n, rows, cols = 30, args.rows, args.cols
     wa, wr = 21, 7
     ha, hr = wa // 2, wr // 2
-    s_opts = dict(window=(wa, wr), shp_test="ks", alpha=0.05, min_shp=30,
-                  estimator="evd", bias=True, pairs="consecutive")
+    if args.config == 2:
+        s_opts = dict(window=(wa, wr), shp_test="ks", alpha=0.05, min_shp=30,
+                      estimator="evd", bias=True, pairs="consecutive")
+        gamma0, label = (0.05, 0.95), "KS alpha 0.05, EVD, min_shp 30, bias correction, " \
+                                      "consecutive pairs"
+    else:
+        s_opts = dict(window=(wa, wr), shp_test="ad", alpha=0.05, min_shp=20,
+                      estimator="emi", bias=True, pairs="all")
+        gamma0, label = (0.6, 0.99), "AD alpha 0.05, EMI (FP64 Cholesky of |T|), " \
+                                     "min_shp 20, bias correction, all 435 pairs"
     s = tcoh.make_settings(n, **s_opts)
 
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x %d bytes read + written" % nb)
+        del src, dst
+
     t0 = time.perf_counter()
-    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0)
+    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
+                                            gamma0=gamma0)
     stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
     slc, amp, valid = prepare_on_device(torch, stack, ha, hr)
     del stack
@@ -218,21 +246,20 @@ def main():
 
     # The dominant kernel is the fused covariance + linking kernel k_link.
     # Algorithmic FLOPs per estimated pixel (L = its SHP count):
-    # covariance 4N(N+1)L, linking (16/3)N^3 + 8N(N-1). For the mixed form
-    # the bound is the matrix-core time of the three split products of the
-    # covariance plus the FP32 vector time of the linking; "peak" is the
+    # covariance 4N(N+1)L, linking (16/3)N^3 + 8N(N-1), and with EMI N^3
+    # FP64 for the Cholesky inverse of |T|. For the mixed form the bound is
+    # the matrix-core time of the three split products of the covariance
+    # plus the FP32 (and FP64) vector time of the linking; "peak" is the
     # FLOP rate at that bound, so frac = bound time / measured time.
     stage_names = ["prep", "shp", "link", "unused"]
     dom = 2
     flop_cov = float(np.sum(4.0 * n * (n + 1) * looks))
     flop_link = est * flops_eigen(n)
-    if args.precision == "plain":
-        t_bound = (flop_cov + flop_link) / PEAK_FP32
-        kname = "k_link plain (FP32 vector ALU)"
-    else:
-        t_bound = 3.0 * flop_cov / PEAK_F16_MFMA + flop_link / PEAK_FP32
-        kname = "k_link %s (split-16-bit MFMA covariance + FP32 linking)" % args.precision
-    flop = flop_cov + flop_link
+    flop_f64 = est * float(n) ** 3 if s_opts["estimator"] == "emi" else 0.0
+    t_bound = 3.0 * flop_cov / PEAK_F16_MFMA + flop_link / PEAK_FP32 + flop_f64 / PEAK_FP64
+    kname = "k_link %s (split-16-bit MFMA covariance + FP32 linking%s)" % (
+        args.precision, ", FP64 Cholesky" if flop_f64 else "")
+    flop = flop_cov + flop_link + flop_f64
     avg_launch_s = st_ms[dom] * 1e-3 / max(1.0, nl[dom])
     achieved = flop / max(1.0, nl[dom]) / avg_launch_s
     peak = flop / t_bound
@@ -242,7 +269,7 @@ def main():
                 "launches_per_step": float(nl[dom])}
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.config == 2 else METRIC_EMI,
         "value": value,
         "unit": "Mpixel/s",
         "n_gpus": 1,
@@ -255,9 +282,9 @@ def main():
         "dtype": "f16 hi/lo split, f32 accumulate" if args.precision == "f16"
                  else "bf16 hi/lo split, f32 accumulate",
         "data": "synthetic",
-        "config": {"workload": "N=30 dates, %dx%d px, 21x7 window, KS alpha 0.05, EVD, "
-                               "min_shp 30, bias correction, consecutive pairs; "
-                               "Voronoi decorrelation scene seed 0" % (rows, cols),
+        "config": {"workload": "N=30 dates, %dx%d px, 21x7 window, %s; Voronoi "
+                               "decorrelation scene (16 cells, gamma0 ~ U(%g, %g)) seed 0"
+                               % (rows, cols, label, gamma0[0], gamma0[1]),
                    "estimated_pixels": est, "mean_shp": float(looks.mean()),
                    "stage_ms": dict(zip(stage_names, np.round(st_ms, 3).tolist()))},
         "roofline": roofline,

@@ -334,6 +334,38 @@ def temporal_coherence(stack, window=(21, 7), shp_test="ks", alpha=0.05,
     return coh, count, pr
 
 
+def geometry_grid(orbits, scene, ref, hmin, hmax, h0=0.0):
+    """tcoh_geometry_init (geometry.c:161-254 on the GPU): the flat-earth /
+    topographic phase grid of a scene.
+
+    orbits: one (t (m,), pos (m, 3), vel (m, 3)) per date; scene: dict
+    with t_first, dt, tau0, fs, lambda, line0, sample0, rows, cols.
+    Returns (phase (grid_rows, grid_cols, nheights, n), heights).
+    """
+    keep = []
+    ob = (Orbit * len(orbits))()
+    for k, (t, pos, vel) in enumerate(orbits):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (t, pos, vel)]
+        keep.append(arrs)
+        ob[k].n = len(arrs[0])
+        ob[k].t = arrs[0].ctypes.data_as(C.POINTER(C.c_double))
+        ob[k].pos = arrs[1].ctypes.data_as(C.POINTER(C.c_double))
+        ob[k].vel = arrs[2].ctypes.data_as(C.POINTER(C.c_double))
+    sc = Scene(scene["t_first"], scene["dt"], scene["tau0"], scene["fs"],
+               scene["lambda"], scene["line0"], scene["sample0"], scene["rows"],
+               scene["cols"])
+    g = Geometry()
+    L = lib()
+    L.tcoh_geometry_init(C.byref(g), C.byref(sc), len(orbits), ref, ob, hmin, hmax, h0)
+    _raise_if_error()
+    n, nh = len(orbits), g.nheights
+    shape = (g.grid_rows, g.grid_cols, nh, n)
+    phase = np.ctypeslib.as_array(g.phase, shape=(int(np.prod(shape)),)).copy()
+    heights = np.array(g.heights[:nh])
+    L.geometry_free(C.byref(g))
+    return phase.reshape(shape), heights
+
+
 def list_devices():
     """ocl_list_devices (ocl.c:105-136), printed to stdout."""
     lib().ocl_list_devices()

@@ -39,6 +39,16 @@
 #define EIG_FLOOR 1.0e-6f
 #define MAX_SWEEPS 40
 
+/* Keep the compiler from hoisting the LDS loads of later steps of an
+   unrolled sequence above this point (it would hold them all in registers
+   and spill them): a compiler-level memory barrier, which emits nothing,
+   and a scheduling barrier. */
+#define TCOH_STEP_FENCE()                                                      \
+    do {                                                                       \
+        asm volatile("" ::: "memory");                                         \
+        __builtin_amdgcn_sched_barrier(0);                                     \
+    } while (0)
+
 /* vw[r] = (v_r, w_r) of the current Householder step, read as one
    16-byte word by the update. vw is dead once the reduction ends, and the
    inverse iteration's LU factors (u0 holds 1/pivot) and result y take its
@@ -57,14 +67,28 @@ template <int N> struct EigLds {
     float2 alpha;
 };
 
-/* Cholesky factor (FP64, lower, column by column) or, in the fallback,
-   the Jacobi matrix and its eigenvectors (FP32). */
+/* Cholesky factor (FP64, lower, column-major packed, with the reciprocal
+   diagonal) or, in the fallback, the Jacobi state (FP32): the rotations of
+   a round, the matrix A and the eigenvectors V (NE x NE, NE = N rounded up
+   to even; the padding row and column stay zero in A). */
+template <int N> struct JacobiShape {
+    static constexpr int NE = (N + 1) & ~1;
+    static constexpr int NP = NE / 2;
+};
+
 template <int N> union __attribute__((aligned(16))) EmiLds {
-    double lc[N * (N + 1) / 2 + 1];
     struct {
-        float a[N * N];
-        float v[N * N];
-        float lam[N];
+        double lc[N * (N + 1) / 2];
+        double rd[N];
+        double flag;
+    } ch;
+    struct {
+        float4 rot[JacobiShape<N>::NP];
+        int4 pq[JacobiShape<N>::NP]; /* p, q, rotation applied */
+        float2 dd[JacobiShape<N>::NP];
+        float il[JacobiShape<N>::NE];
+        float a[JacobiShape<N>::NE * JacobiShape<N>::NE];
+        float v[JacobiShape<N>::NE * JacobiShape<N>::NE];
     } jac;
 };
 
@@ -169,8 +193,12 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             wc[m] = make_float2(0.f, 0.f);
             if (min(m * G + G - 1, N - 1) < k + 1)
                 continue;
+            /* constant trip counts (predicated) keep every index of A
+               static once the step loop is unrolled */
 #pragma unroll
-            for (int j = k + 1; j < N; j++) {
+            for (int j = 0; j < N; j++) {
+                if (j <= k)
+                    continue;
                 const float4 vw = sh.vw[j];
                 const float2 v = make_float2(vw.x, vw.y);
 
@@ -204,6 +232,7 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             }
         }
         wave_sync();
+        TCOH_STEP_FENCE();
 
         /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r > k. */
 #pragma unroll
@@ -211,7 +240,9 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             if (min(m * G + G - 1, N - 1) < k + 1)
                 continue;
 #pragma unroll
-            for (int r = k + 1; r < N; r++) {
+            for (int r = 0; r < N; r++) {
+                if (r <= k)
+                    continue;
                 const float4 vw = sh.vw[r];
                 const float2 v = make_float2(vw.x, vw.y), w = make_float2(vw.z, vw.w);
                 float ax = A[m][r].x, ay = A[m][r].y;
@@ -450,9 +481,9 @@ __device__ __forceinline__ float2 load_t(const float2 *__restrict__ tp, int r,
     return make_float2(t.x, -t.y);
 }
 
-/* |T|^-1 into inv (owned columns) for EMI. */
+/* |T|^-1 into inv (owned columns) for EMI (tcoh.cl:657-687). */
 template <int N, int G, int CPL>
-__device__ void abs_inverse(const float2 *__restrict__ tp, bool est,
+__device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool est,
                             EmiLds<N> &em, int lg, float (&inv)[CPL][N])
 {
     double g[CPL][N];
@@ -471,7 +502,8 @@ __device__ void abs_inverse(const float2 *__restrict__ tp, bool est,
             g[m][r] = (double)hypotf(t.x, t.y);
         }
     }
-    /* Right-looking Cholesky; column j of L goes to LDS. */
+    /* Right-looking Cholesky in FP64 with the reference's pivot floor
+       (tcoh.cl:446-492); column j of L and 1 / L_jj go to LDS. */
 #pragma unroll
     for (int j = 0; j < N; j++) {
 #pragma unroll
@@ -481,159 +513,281 @@ __device__ void abs_inverse(const float2 *__restrict__ tp, bool est,
             if (c == j) {
                 const double s = g[m][j];
                 const double ljj = sqrt(s > 0.0 ? s : 1.0);
+                const double rl = 1.0 / ljj;
 
-                em.lc[N * (N + 1) / 2] = s > CHOL_FLOOR ? 1.0 : 0.0;
-                em.lc[lpos<N>(j, j)] = ljj;
+                em.ch.flag = s > CHOL_FLOOR ? 1.0 : 0.0;
+                em.ch.lc[lpos<N>(j, j)] = ljj;
+                em.ch.rd[j] = rl;
+                /* constant trip counts (predicated) keep every index of g
+                   static once the outer loop is unrolled */
 #pragma unroll
-                for (int i = j + 1; i < N; i++)
-                    em.lc[lpos<N>(i, j)] = g[m][i] / ljj;
+                for (int i = 0; i < N; i++)
+                    if (i > j)
+                        em.ch.lc[lpos<N>(i, j)] = g[m][i] * rl;
             }
         }
         wave_sync();
-        ok = ok && em.lc[N * (N + 1) / 2] != 0.0;
+        ok = ok && em.ch.flag != 0.0;
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
             const int c = lg + m * G;
 
             if (c > j && c < N) {
-                const double lcj = em.lc[lpos<N>(c, j)];
+                const double lcj = em.ch.lc[lpos<N>(c, j)];
 
 #pragma unroll
-                for (int r = j + 1; r < N; r++)
-                    if (r >= c)
-                        g[m][r] -= em.lc[lpos<N>(r, j)] * lcj;
+                for (int r = 0; r < N; r++)
+                    if (r > j && r >= c)
+                        g[m][r] -= em.ch.lc[lpos<N>(r, j)] * lcj;
             }
         }
         wave_sync();
+        TCOH_STEP_FENCE();
     }
     if (ok) {
-        /* Column c of the inverse: L y = e_c, then L^T x = y. */
-#pragma unroll
-        for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
-            double yv[N];
+        /* Column c of the inverse: L y = e_c by columns of L (axpy form),
+           then L^T x = y by dot products with the columns of L; all owned
+           columns advance together, so each entry of L read from LDS is
+           used at once by every slot. */
+        double y[CPL][N];
 
 #pragma unroll
-            for (int r = 0; r < N; r++) {
-                double s = r == c ? 1.0 : 0.0;
-
-#pragma unroll
-                for (int k = 0; k < r; k++)
-                    s -= em.lc[lpos<N>(r, k)] * yv[k];
-                yv[r] = s / em.lc[lpos<N>(r, r)];
-            }
-#pragma unroll
-            for (int r = N - 1; r >= 0; r--) {
-                double s = yv[r];
-
-#pragma unroll
-                for (int k = r + 1; k < N; k++)
-                    s -= em.lc[lpos<N>(k, r)] * yv[k];
-                yv[r] = s / em.lc[lpos<N>(r, r)];
-            }
+        for (int m = 0; m < CPL; m++)
 #pragma unroll
             for (int r = 0; r < N; r++)
-                inv[m][r] = c < N ? (float)yv[r] : 0.0f;
+                y[m][r] = r == lg + m * G ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const double rk = em.ch.rd[k];
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+                y[m][k] *= rk;
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                if (r <= k)
+                    continue;
+                const double l = em.ch.lc[lpos<N>(r, k)];
+
+#pragma unroll
+                for (int m = 0; m < CPL; m++)
+                    y[m][r] -= l * y[m][k];
+            }
+            TCOH_STEP_FENCE();
         }
+#pragma unroll
+        for (int r = N - 1; r >= 0; r--) {
+            double sm[CPL];
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+                sm[m] = y[m][r];
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                if (k <= r)
+                    continue;
+                const double l = em.ch.lc[lpos<N>(k, r)];
+
+#pragma unroll
+                for (int m = 0; m < CPL; m++)
+                    sm[m] -= l * y[m][k];
+            }
+            const double rr = em.ch.rd[r];
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+                y[m][r] = sm[m] * rr;
+            TCOH_STEP_FENCE();
+        }
+#pragma unroll
+        for (int m = 0; m < CPL; m++)
+#pragma unroll
+            for (int r = 0; r < N; r++)
+                inv[m][r] = lg + m * G < N ? (float)y[m][r] : 0.0f;
     }
     wave_sync();
     if (ok)
         return;
 
-    /* Not safely positive definite: pseudo-inverse from a cyclic Jacobi
-       eigendecomposition of |T| in FP32 (tcoh.cl:97-199, 663-687); |T| is
+    /* Not safely positive definite: pseudo-inverse from a Jacobi
+       eigendecomposition of |T| in FP32 with the reference's rotations,
+       stopping rule and sweep limit (tcoh.cl:97-199, 663-687), in the
+       parallel ordering: each round rotates NP disjoint pairs (round-robin
+       tournament; a zero index pads odd N), one pair per lane, then updates
+       the 2 x 2 blocks of A and the column pairs of V in parallel. |T| is
        real, so the phase step of the complex rotation is a sign. */
+    typedef JacobiShape<N> J;
+    constexpr int NE = J::NE, NP = J::NP, NB = NP * (NP + 1) / 2;
     float *a = em.jac.a, *v = em.jac.v;
 
-    for (int i = lg; i < N * N; i += G) {
-        const int r = i / N, c = i % N;
+    for (int i = lg; i < NE * NE; i += G) {
+        const int r = i / NE, c = i - (i / NE) * NE;
         float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
 
-        if (est)
+        if (r < N && c < N && est)
             t = load_t<N>(tp, r, c);
-        a[i] = hypotf(t.x, t.y);
+        a[i] = (r < N && c < N) ? hypotf(t.x, t.y) : 0.0f;
         v[i] = r == c ? 1.0f : 0.0f;
     }
     wave_sync();
     for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
         float off = 0.0f, dg = 0.0f;
 
-        for (int p = 0; p < N; p++) {
-            dg += a[p * N + p] * a[p * N + p];
-            for (int q = p + 1; q < N; q++)
-                off += a[p * N + q] * a[p * N + q];
+        for (int i = lg; i < N * N; i += G) {
+            const int r = i / N, c = i - (i / N) * N;
+            const float x = a[r * NE + c];
+
+            if (r == c)
+                dg += x * x;
+            else if (r < c)
+                off += x * x;
         }
+        off = gsum<G>(off);
+        dg = gsum<G>(dg);
         if (off <= 1.0e-14f * dg)
             break;
-        for (int p = 0; p < N - 1; p++) {
-            for (int q = p + 1; q < N; q++) {
-                const float apq = a[p * N + q];
+        for (int round = 0; round < NE - 1; round++) {
+            /* The rotation of pair lg: circle positions lg and NE-1-lg,
+               position 0 fixed, the others advanced by the round. */
+            if (lg < NP) {
+                const int s2 = NE - 1 - lg;
+                const int i1 = lg == 0 ? 0 : (lg - 1 + round) % (NE - 1) + 1;
+                const int i2 = (s2 - 1 + round) % (NE - 1) + 1;
+                const int p = min(i1, i2), q = max(i1, i2);
+                const float apq = a[p * NE + q];
                 const float r = fabsf(apq);
+                float4 jr = make_float4(1.f, 0.f, 0.f, 1.f);
+                float2 d = make_float2(0.f, 0.f);
 
-                if (r == 0.0f)
-                    continue;
-                const float sg = apq < 0.0f ? -1.0f : 1.0f;
-                const float theta = (a[q * N + q] - a[p * N + p]) / (2.0f * r);
-                float t;
+                if (r != 0.0f) {
+                    const float sg = apq < 0.0f ? -1.0f : 1.0f;
+                    const float app = a[p * NE + p], aqq = a[q * NE + q];
+                    const float theta = (aqq - app) / (2.0f * r);
+                    float t;
 
-                if (fabsf(theta) > 1.0e18f)
-                    t = 0.5f / theta;
-                else
-                    t = (theta >= 0.0f ? 1.0f : -1.0f) /
-                        (fabsf(theta) + sqrtf(theta * theta + 1.0f));
-                const float cs = 1.0f / sqrtf(t * t + 1.0f), sn = t * cs;
-                const float app = a[p * N + p] - t * r;
-                const float aqq = a[q * N + q] + t * r;
-                wave_sync();
-                for (int k = lg; k < N; k += G) {
-                    if (k != p && k != q) {
-                        const float kp = a[k * N + p], kq = sg * a[k * N + q];
+                    if (fabsf(theta) > 1.0e18f)
+                        t = 0.5f / theta;
+                    else
+                        t = (theta >= 0.0f ? 1.0f : -1.0f) /
+                            (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+                    const float cs = 1.0f / sqrtf(t * t + 1.0f), sn = t * cs;
 
-                        a[k * N + p] = a[p * N + k] = cs * kp - sn * kq;
-                        a[k * N + q] = a[q * N + k] = sn * kp + cs * kq;
-                    }
-                    const float vp = v[k * N + p], vq = sg * v[k * N + q];
-
-                    v[k * N + p] = cs * vp - sn * vq;
-                    v[k * N + q] = sn * vp + cs * vq;
+                    /* new col p = cs p - sn sg q, new col q = sn p + cs sg q */
+                    jr = make_float4(cs, sn, -sn * sg, cs * sg);
+                    d = make_float2(app - t * r, aqq + t * r);
                 }
-                wave_sync();
-                if (lg == 0) {
-                    a[p * N + p] = app;
-                    a[q * N + q] = aqq;
-                    a[p * N + q] = a[q * N + p] = 0.0f;
-                }
-                wave_sync();
+                em.jac.rot[lg] = jr;
+                em.jac.dd[lg] = d;
+                em.jac.pq[lg] = make_int4(p, q, r != 0.0f, 0);
             }
+            wave_sync();
+            /* A <- J^T A J, one upper 2 x 2 block (i, k) at a time. */
+            {
+                int i = 0, rem = lg;
+
+                while (i < NP && rem >= NP - i) {
+                    rem -= NP - i;
+                    i++;
+                }
+                for (int b = lg; b < NB; b += G) {
+                    const int k = i + rem;
+                    const int4 pi = em.jac.pq[i], pk = em.jac.pq[k];
+
+                    if (i == k) {
+                        if (pi.z) {
+                            const float2 d = em.jac.dd[i];
+
+                            a[pi.x * NE + pi.x] = d.x;
+                            a[pi.y * NE + pi.y] = d.y;
+                            a[pi.x * NE + pi.y] = 0.0f;
+                            a[pi.y * NE + pi.x] = 0.0f;
+                        }
+                    }
+                    else if (pi.z || pk.z) {
+                        const float4 ri = em.jac.rot[i], rk = em.jac.rot[k];
+                        const float m00 = a[pi.x * NE + pk.x], m01 = a[pi.x * NE + pk.y];
+                        const float m10 = a[pi.y * NE + pk.x], m11 = a[pi.y * NE + pk.y];
+                        /* columns (pk, qk) times J_k, then rows (pi, qi) by J_i^T */
+                        const float r00 = m00 * rk.x + m01 * rk.z, r01 = m00 * rk.y + m01 * rk.w;
+                        const float r10 = m10 * rk.x + m11 * rk.z, r11 = m10 * rk.y + m11 * rk.w;
+                        const float n00 = ri.x * r00 + ri.z * r10, n01 = ri.x * r01 + ri.z * r11;
+                        const float n10 = ri.y * r00 + ri.w * r10, n11 = ri.y * r01 + ri.w * r11;
+
+                        a[pi.x * NE + pk.x] = a[pk.x * NE + pi.x] = n00;
+                        a[pi.x * NE + pk.y] = a[pk.y * NE + pi.x] = n01;
+                        a[pi.y * NE + pk.x] = a[pk.x * NE + pi.y] = n10;
+                        a[pi.y * NE + pk.y] = a[pk.y * NE + pi.y] = n11;
+                    }
+                    rem += G;
+                    while (i < NP && rem >= NP - i) {
+                        rem -= NP - i;
+                        i++;
+                    }
+                }
+            }
+            /* V <- V J on the real rows. */
+            for (int e = lg; e < N * NP; e += G) {
+                const int r = e / NP, i = e - (e / NP) * NP;
+                const int4 pi = em.jac.pq[i];
+
+                if (pi.z) {
+                    const float4 ri = em.jac.rot[i];
+                    const float vp = v[r * NE + pi.x], vq = v[r * NE + pi.y];
+
+                    v[r * NE + pi.x] = vp * ri.x + vq * ri.z;
+                    v[r * NE + pi.y] = vp * ri.y + vq * ri.w;
+                }
+            }
+            wave_sync();
         }
     }
-    {
-        float lmax = 0.0f;
 
-        for (int k = 0; k < N; k++)
-            lmax = fmaxf(lmax, fabsf(a[k * N + k]));
-        lmax *= EIG_FLOOR;
+    /* inv = V diag(1 / lambda) V^T with eigenvalues below 1e-6 of the
+       largest magnitude dropped (tcoh.cl:669-677). */
+    float lmax = 0.0f;
+
+    for (int k = lg; k < N; k += G)
+        lmax = fmaxf(lmax, fabsf(a[k * NE + k]));
+    lmax = gmax<G>(lmax) * EIG_FLOOR;
+    for (int k = lg; k < N; k += G) {
+        const float lk = a[k * NE + k];
+
+        em.jac.il[k] = fabsf(lk) < lmax ? 0.0f : 1.0f / lk;
+    }
+    wave_sync();
+    {
+        float w[CPL][N];
+
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
             const int c = lg + m * G;
 
 #pragma unroll
-            for (int r = 0; r < N; r++) {
-                float s = 0.0f;
+            for (int k = 0; k < N; k++)
+                w[m][k] = c < N ? v[c * NE + k] * em.jac.il[k] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            float sm[CPL];
 
-                if (c < N)
-                    for (int k = 0; k < N; k++) {
-                        const float lk = a[k * N + k];
-                        const float il = fabsf(lk) < lmax ? 0.0f : 1.0f / lk;
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+                sm[m] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                const float x = v[r * NE + k];
 
-                        s += v[r * N + k] * v[c * N + k] * il;
-                    }
-                inv[m][r] = s;
+#pragma unroll
+                for (int m = 0; m < CPL; m++)
+                    sm[m] = fmaf(x, w[m][k], sm[m]);
             }
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+                inv[m][r] = sm[m];
+            TCOH_STEP_FENCE();
         }
     }
     wave_sync();
 }
-
 
 #endif

@@ -36,7 +36,7 @@ template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
 
 /* COV: 0 plain FP32, 1 split FP16 MFMA, 2 split BF16 MFMA. */
 template <int N, int COV, int EMI>
-__global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
+__global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
                                                const uint2 *__restrict__ hl,
                                                const uint32_t *__restrict__ mask,

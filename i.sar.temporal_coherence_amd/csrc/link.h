@@ -19,15 +19,19 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     constexpr int CPL = (N + G - 1) / G;
     float2 A[CPL][N];
 
+    /* EVD keeps T in registers from here on; EMI reads it again after the
+       inversion, so that it is not held live through it. */
+    if (!EMI) {
 #pragma unroll
-    for (int m = 0; m < CPL; m++) {
-        const int c = lg + m * G;
+        for (int m = 0; m < CPL; m++) {
+            const int c = lg + m * G;
 
 #pragma unroll
-        for (int r = 0; r < N; r++) {
-            A[m][r] = make_float2(r == c ? 1.f : 0.f, 0.f);
-            if (est && c < N)
-                A[m][r] = load_t<N>(tp, r, c);
+            for (int r = 0; r < N; r++) {
+                A[m][r] = make_float2(r == c ? 1.f : 0.f, 0.f);
+                if (est && c < N)
+                    A[m][r] = load_t<N>(tp, r, c);
+            }
         }
     }
 
@@ -44,9 +48,14 @@ __device__ __forceinline__ float link_group(const KParams &kp,
 
             adj[m] = 0ull;
 #pragma unroll
-            for (int r = 0; r < N; r++)
-                if (c < N && r != c && (A[m][r].x != 0.0f || A[m][r].y != 0.0f))
+            for (int r = 0; r < N; r++) {
+                const float2 t = EMI ? (est && c < N ? load_t<N>(tp, r, c)
+                                                     : make_float2(0.f, 0.f))
+                                     : A[m][r];
+
+                if (c < N && r != c && (t.x != 0.0f || t.y != 0.0f))
                     adj[m] |= 1ull << r;
+            }
         }
         for (int it = 0; it < N; it++) {
             unsigned long long add = 0ull;

@@ -150,3 +150,42 @@ def config_voronoi(n=30, rows=2048, cols=2048, seed=0, gamma0=(0.05, 0.95),
     """Region map and parameters of the Voronoi scenes (16 cells)."""
     rng = np.random.default_rng(seed + 1000)
     return voronoi_regions(rows, cols, cells, rng, gamma0, tau)
+
+
+# ---- acquisition geometry (after tests/geometry_model.py:15-24) ----------
+
+LIGHT = 299792458.0
+ELLIPSOID_A = 6378137.0
+
+
+def circular_orbits(t0s, baselines, radius=ELLIPSOID_A + 700e3, speed=7500.0,
+                    latitude=25.0, sv_dt=10.0, half_span=60.0):
+    """State vectors of circular polar orbits in an Earth-fixed frame over
+    longitude 0, one per date: orbit k crosses `latitude` (degrees) at t0s[k]
+    heading north, offset across track (+y) by baselines[k]; state vectors
+    every sv_dt seconds within t0 +- half_span. Returns [(t, pos, vel)]."""
+    omega = speed / radius
+    out = []
+    for t0, b in zip(t0s, baselines):
+        t = t0 + np.arange(-half_span, half_span + 0.5 * sv_dt, sv_dt)
+        th = math.radians(latitude) + omega * (t - t0)
+        pos = np.stack([radius * np.cos(th), np.full_like(th, b), radius * np.sin(th)], 1)
+        vel = np.stack([-radius * omega * np.sin(th), np.zeros_like(th),
+                        radius * omega * np.cos(th)], 1)
+        out.append((t, pos, vel))
+    return out
+
+
+def geometry_case(n=5, rows=300, cols=500, ref=None, seed=0, days=12.0):
+    """A C-band stack geometry: dates every `days`, baselines ~ N(0, 100 m)
+    (0 at the reference date), near range 850 km, the reference date's
+    raster timing. Returns (orbits, scene dict, ref)."""
+    rng = np.random.default_rng(seed)
+    ref = n // 2 if ref is None else ref
+    t0s = 1.0e5 + 86400.0 * days * np.arange(n)
+    base = rng.normal(0.0, 100.0, n)
+    base[ref] = 0.0
+    scene = dict(t_first=t0s[ref], dt=2.055556e-3, tau0=2.0 * 850e3 / LIGHT,
+                 fs=64.34523812571427e6, **{"lambda": LIGHT / 5.405e9},
+                 line0=0.0, sample0=0.0, rows=rows, cols=cols)
+    return circular_orbits(t0s, base), scene, ref

@@ -151,3 +151,25 @@ def prepare_strip(i, q, row0, padded_rows, halo_cols, phase=None, elev_null=None
                                None if en is None else en.ctypes.data, row0,
                                padded_rows, halo_cols, slc, amp, valid)
     return slc, amp, valid
+
+
+def geometry_grid(orbits, scene, ref, hmin, hmax):
+    """geometry.c:161-254 in FP64 on the CPU: the phase grid (grid_rows,
+    grid_cols, nheights, n) and heights; orbits and scene as
+    tcoh_amd.api.geometry_grid takes them."""
+    n = len(orbits)
+    nsv = np.array([len(o[0]) for o in orbits], np.int32)
+    off = np.concatenate([[0], np.cumsum(nsv)[:-1]]).astype(np.int32)
+    t = np.ascontiguousarray(np.concatenate([o[0] for o in orbits]), np.float64)
+    pos = np.ascontiguousarray(np.concatenate([o[1] for o in orbits]), np.float64).reshape(-1)
+    vel = np.ascontiguousarray(np.concatenate([o[2] for o in orbits]), np.float64).reshape(-1)
+    gr, gc, nh = C.c_int(), C.c_int(), C.c_int()
+    hs = np.zeros(3)
+    args = [n, ref, nsv, off, t, pos, vel, scene["t_first"], scene["dt"], scene["tau0"],
+            scene["fs"], scene["lambda"], scene["line0"], scene["sample0"],
+            scene["rows"], scene["cols"], hmin, hmax]
+    lib().oracle_geometry_init(*args, None, C.byref(gr), C.byref(gc), C.byref(nh), hs)
+    phase = np.zeros(gr.value * gc.value * nh.value * n)
+    status = lib().oracle_geometry_init(*args, phase.ctypes.data, C.byref(gr),
+                                        C.byref(gc), C.byref(nh), hs)
+    return phase.reshape(gr.value, gc.value, nh.value, n), hs[:nh.value], status

@@ -148,3 +148,22 @@ def test_errors_are_reported(tcoh):
         with pytest.raises(RuntimeError, match="padded width"):
             be.run(np.zeros(30 * 22 * 12, np.float32), np.zeros(30 * 22 * 6, np.float32),
                    np.zeros(30 * 22, np.uint8), 30, 20, 20)
+
+
+def test_geometry_grid_matches_oracle(tcoh, cref):
+    """tcoh_geometry_init (FP64 on the GPU) against the FP64 C restatement
+    of geometry.c:161-254: the same Newton iterations in the same
+    precision, so the grids agree to far below the stopping rules' effect
+    (geolocation steps < 1e-6 m: 4 pi / lambda x 1e-6 m = 2.3e-4 rad);
+    asserted 1e-6 rad, on three heights and a reference date in the
+    middle."""
+    orbits, scene, ref = tcoh.synth.geometry_case(n=6, rows=400, cols=700, seed=2)
+    got, hs = tcoh.api.geometry_grid(orbits, scene, ref, 10.0, 900.0)
+    want, ohs, status = cref.geometry_grid(orbits, scene, ref, 10.0, 900.0)
+    assert status == 0 and got.shape == want.shape and np.array_equal(hs, ohs)
+    assert np.all(got[..., ref] == 0.0)
+    err = np.max(np.abs(got - want))
+    assert err <= 1e-6, "max |d phase| %.3g rad" % err
+    # under 1 m of relief: one height, the middle (geometry.c:204-213)
+    flat, fhs = tcoh.api.geometry_grid(orbits, scene, ref, 100.0, 100.5)
+    assert flat.shape[2] == 1 and fhs[0] == 100.25
