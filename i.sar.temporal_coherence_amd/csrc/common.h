@@ -65,6 +65,13 @@ __device__ __forceinline__ float rcpf(float x)
     return __builtin_amdgcn_rcpf(x);
 }
 
+/* a / b: correctly rounded when EXACT (the reference's division), else
+   through the hardware reciprocal. */
+template <bool EXACT> __device__ __forceinline__ float qdiv(float a, float b)
+{
+    return EXACT ? a / b : a * rcpf(b);
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

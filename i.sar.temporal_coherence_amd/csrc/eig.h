@@ -81,6 +81,7 @@ template <int N> union __attribute__((aligned(16))) EmiLds {
         double lc[N * (N + 1) / 2];
         double rd[N];
         double flag;
+        double w[N > 32 ? N * (N + 1) / 2 : 1]; /* L^-1 (N > 32) */
     } ch;
     struct {
         float4 rot[JacobiShape<N>::NP];
@@ -107,7 +108,7 @@ template <int N> __device__ __forceinline__ int lpos(int i, int j)
    a live slot (c <= k) are updated like the others: their rows below k
    are never read again (this layout keeps Householder vectors in rows,
    not columns), so only the reductions and LDS writes are masked. */
-template <int N, int G, int CPL>
+template <int N, int G, int CPL, bool EXACT>
 __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                                                EigLds<N> &sh, int lg)
 {
@@ -138,16 +139,14 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             beta = alr;
         else {
             beta = -copysignf(sqrtf(alr * alr + ali * ali + xn), alr);
-            const float rb = rcpf(beta);
-
-            tr_ = (beta - alr) * rb;
-            ti_ = -ali * rb;
+            tr_ = qdiv<EXACT>(beta - alr, beta);
+            ti_ = qdiv<EXACT>(-ali, beta);
             {
                 const float dr = alr - beta;
-                const float rd = rcpf(dr * dr + ali * ali);
+                const float den = dr * dr + ali * ali;
 
-                sr = dr * rd;
-                si = -ali * rd;
+                sr = qdiv<EXACT>(dr, den);
+                si = qdiv<EXACT>(-ali, den);
             }
         }
         float2 vc[CPL];
@@ -316,7 +315,7 @@ __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
 
 /* Inverse iteration of tcoh.cl:340-410; every lane of the group computes
    the same vector, y ends in LDS. */
-template <int N>
+template <int N, bool EXACT>
 __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
                                                     float lambda, float norm,
                                                     int lg)
@@ -335,7 +334,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
         if (fabsf(dd) >= fabsf(sub)) {
             if (fabsf(dd) < tiny)
                 dd = dd < 0.0f ? -tiny : tiny;
-            lk = sub * rcpf(dd);
+            lk = qdiv<EXACT>(sub, dd);
             pv = 0;
             a0 = dd;
             a1 = s1;
@@ -344,7 +343,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
             s1 = ns;
         }
         else {
-            lk = dd * rcpf(sub);
+            lk = qdiv<EXACT>(dd, sub);
             pv = 1;
             a0 = sub;
             a1 = nd;
@@ -354,13 +353,13 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
         }
         sh.l[k] = lk;
         sh.piv[k] = pv;
-        sh.u0[k] = rcpf(a0);
+        sh.u0[k] = EXACT ? a0 : rcpf(a0);
         sh.u1[k] = a1;
         sh.u2[k] = a2;
     }
     if (fabsf(dd) < tiny)
         dd = dd < 0.0f ? -tiny : tiny;
-    sh.u0[N - 1] = rcpf(dd);
+    sh.u0[N - 1] = EXACT ? dd : rcpf(dd);
     sh.u1[N - 1] = 0.0f;
     sh.u2[N - 1] = 0.0f;
 
@@ -388,7 +387,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
                 v -= sh.u1[k] * y[k + 1];
             if (k + 2 < N)
                 v -= sh.u2[k] * y[k + 2];
-            y[k] = v * sh.u0[k];
+            y[k] = EXACT ? v / sh.u0[k] : v * sh.u0[k];
         }
 #pragma unroll
         for (int k = 0; k < N; k++)
@@ -397,7 +396,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
 
 #pragma unroll
         for (int k = 0; k < N; k++)
-            y[k] *= rn;
+            y[k] = EXACT ? y[k] / nrm : y[k] * rn;
     }
     if (lg == 0) {
 #pragma unroll
@@ -454,16 +453,16 @@ __device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
 
 /* Eigenvector (owned components in z) of rank idx of the group's
    Hermitian matrix A (destroyed). */
-template <int N, int G, int CPL>
+template <int N, int G, int CPL, bool EXACT>
 __device__ __forceinline__ void hermitian_eigenvector(float2 (&A)[CPL][N],
                                                       EigLds<N> &sh, int idx,
                                                       int lg, float2 (&z)[CPL])
 {
     float norm;
 
-    tridiagonalize<N, G, CPL>(A, sh, lg);
+    tridiagonalize<N, G, CPL, EXACT>(A, sh, lg);
     const float lambda = tridiag_eigenvalue<N, G>(sh, idx, lg, &norm);
-    tridiag_eigenvector<N>(sh, lambda, norm, lg);
+    tridiag_eigenvector<N, EXACT>(sh, lambda, norm, lg);
     back_transform<N, G, CPL>(A, sh, lg, z);
 }
 
@@ -486,8 +485,78 @@ template <int N, int G, int CPL>
 __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool est,
                             EmiLds<N> &em, int lg, float (&inv)[CPL][N])
 {
-    double g[CPL][N];
     bool ok = true;
+
+    if constexpr (N > 32) {
+        /* One column per lane (G = 64): L, L^-1 and the pivots in LDS with
+           rolled loops (a register-resident factor of this size would hold
+           N^2 doubles per lane in unrolled code). Lane c factors row c of
+           |T| and then computes column c of W = L^-1; |T|^-1 = W^T W. */
+        static_assert(CPL == 1, "one column per lane");
+        double *l = em.ch.lc, *w = em.ch.w, *rd = em.ch.rd;
+        const int c = lg;
+
+        if (c < N)
+            for (int j = 0; j <= c; j++) {
+                const float2 t = est ? load_t<N>(tp, c, j)
+                                     : make_float2(c == j ? 1.f : 0.f, 0.f);
+
+                l[lpos<N>(c, j)] = (double)hypotf(t.x, t.y);
+            }
+        for (int j = 0; j < N; j++) {
+            wave_sync();
+            const double s = l[lpos<N>(j, j)];
+
+            if (!(s > CHOL_FLOOR)) {
+                ok = false;
+                break;
+            }
+            const double ljj = sqrt(s), rl = 1.0 / ljj;
+
+            wave_sync();
+            if (c == j) {
+                l[lpos<N>(j, j)] = ljj;
+                rd[j] = rl;
+            }
+            if (c > j && c < N)
+                l[lpos<N>(c, j)] *= rl;
+            wave_sync();
+            if (c > j && c < N) {
+                const double lcj = l[lpos<N>(c, j)];
+
+                for (int k = j + 1; k <= c; k++)
+                    l[lpos<N>(c, k)] -= lcj * l[lpos<N>(k, j)];
+            }
+        }
+        wave_sync();
+        if (ok) {
+            if (c < N) {
+                w[lpos<N>(c, c)] = rd[c];
+                for (int r = c + 1; r < N; r++) {
+                    double sm = 0.0;
+
+                    for (int k = c; k < r; k++)
+                        sm += l[lpos<N>(r, k)] * w[lpos<N>(k, c)];
+                    w[lpos<N>(r, c)] = -sm * rd[r];
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                double sm = 0.0;
+
+                if (c < N)
+                    for (int k = max(r, c); k < N; k++)
+                        sm += w[lpos<N>(k, r)] * w[lpos<N>(k, c)];
+                inv[0][r] = (float)sm;
+            }
+        }
+        wave_sync();
+        if (ok)
+            return;
+    }
+    else {
+    double g[CPL][N];
 
 #pragma unroll
     for (int m = 0; m < CPL; m++) {
@@ -608,60 +677,87 @@ __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool 
     wave_sync();
     if (ok)
         return;
+    }
 
     /* Not safely positive definite: pseudo-inverse from a Jacobi
        eigendecomposition of |T| in FP32 with the reference's rotations,
        stopping rule and sweep limit (tcoh.cl:97-199, 663-687), in the
-       parallel ordering: each round rotates NP disjoint pairs (round-robin
-       tournament; a zero index pads odd N), one pair per lane, then updates
-       the 2 x 2 blocks of A and the column pairs of V in parallel. |T| is
-       real, so the phase step of the complex rotation is a sign. */
+       parallel (round-robin tournament) ordering, systolic: lane i < NP
+       holds the two columns of A and of V at tournament positions i and
+       NE-1-i (a zero index pads odd N). A is kept in position coordinates
+       (rows permuted with the columns), so every row index below is
+       static. Per round each lane rotates its own pair's columns, all
+       pairs' row rotations are applied from a broadcast, and the columns
+       move one tournament position (wavefront shifts). |T| is real, so
+       the phase step of the complex rotation is a sign. */
     typedef JacobiShape<N> J;
-    constexpr int NE = J::NE, NP = J::NP, NB = NP * (NP + 1) / 2;
+    constexpr int NE = J::NE, NP = J::NP;
     float *a = em.jac.a, *v = em.jac.v;
+    const int uL = lg, uR = NE - 1 - lg; /* this lane's positions */
+    const bool act = lg < NP;
+    float cL[NE], cR[NE], vL[N], vR[N];
+    int iL = act ? uL : 0, iR = act ? uR : 0; /* matrix index at each */
 
-    for (int i = lg; i < NE * NE; i += G) {
-        const int r = i / NE, c = i - (i / NE) * NE;
-        float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < NE; r++) {
+        float2 tl_ = make_float2(r == iL ? 1.f : 0.f, 0.f);
+        float2 tr_ = make_float2(r == iR ? 1.f : 0.f, 0.f);
 
-        if (r < N && c < N && est)
-            t = load_t<N>(tp, r, c);
-        a[i] = (r < N && c < N) ? hypotf(t.x, t.y) : 0.0f;
-        v[i] = r == c ? 1.0f : 0.0f;
+        if (est && r < N && iL < N && act)
+            tl_ = load_t<N>(tp, r, iL);
+        if (est && r < N && iR < N && act)
+            tr_ = load_t<N>(tp, r, iR);
+        cL[r] = (r < N && iL < N) ? hypotf(tl_.x, tl_.y) : 0.0f;
+        cR[r] = (r < N && iR < N) ? hypotf(tr_.x, tr_.y) : 0.0f;
     }
-    wave_sync();
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+        vL[r] = r == iL ? 1.0f : 0.0f;
+        vR[r] = r == iR ? 1.0f : 0.0f;
+    }
+    /* Dynamic position reads as select chains over static registers. */
+    auto at = [&](const float(&x)[NE], int u) {
+        float o = 0.0f;
+
+#pragma unroll
+        for (int r = 0; r < NE; r++)
+            o = r == u ? x[r] : o;
+        return o;
+    };
     for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
         float off = 0.0f, dg = 0.0f;
 
-        for (int i = lg; i < N * N; i += G) {
-            const int r = i / N, c = i - (i / N) * N;
-            const float x = a[r * NE + c];
+        /* off-diagonal squares summed directly (never as a difference of
+           two large sums): each pair p < q appears in two columns */
+        if (act) {
+#pragma unroll
+            for (int r = 0; r < NE; r++) {
+                off += r == uL ? 0.0f : cL[r] * cL[r];
+                off += r == uR ? 0.0f : cR[r] * cR[r];
+            }
+            const float dl = at(cL, uL), dr = at(cR, uR);
 
-            if (r == c)
-                dg += x * x;
-            else if (r < c)
-                off += x * x;
+            dg = fmaf(dl, dl, dr * dr);
         }
-        off = gsum<G>(off);
+        off = 0.5f * gsum<G>(off);
         dg = gsum<G>(dg);
         if (off <= 1.0e-14f * dg)
             break;
         for (int round = 0; round < NE - 1; round++) {
-            /* The rotation of pair lg: circle positions lg and NE-1-lg,
-               position 0 fixed, the others advanced by the round. */
-            if (lg < NP) {
-                const int s2 = NE - 1 - lg;
-                const int i1 = lg == 0 ? 0 : (lg - 1 + round) % (NE - 1) + 1;
-                const int i2 = (s2 - 1 + round) % (NE - 1) + 1;
-                const int p = min(i1, i2), q = max(i1, i2);
-                const float apq = a[p * NE + q];
-                const float r = fabsf(apq);
-                float4 jr = make_float4(1.f, 0.f, 0.f, 1.f);
-                float2 d = make_float2(0.f, 0.f);
+            /* Rotation of this lane's pair in position terms:
+               new col uL = j.x col uL + j.z col uR,
+               new col uR = j.y col uL + j.w col uR. */
+            float4 jr = make_float4(1.f, 0.f, 0.f, 1.f);
+            float dpp_ = 0.f, dqq_ = 0.f;
+            bool rot = false, lp = iL < iR; /* L holds the lower index */
+
+            if (act) {
+                const float aLL = at(cL, uL), aRR = at(cR, uR), aLR = at(cL, uR);
+                const float app = lp ? aLL : aRR, aqq = lp ? aRR : aLL;
+                const float r = fabsf(aLR);
 
                 if (r != 0.0f) {
-                    const float sg = apq < 0.0f ? -1.0f : 1.0f;
-                    const float app = a[p * NE + p], aqq = a[q * NE + q];
+                    const float sg = aLR < 0.0f ? -1.0f : 1.0f;
                     const float theta = (aqq - app) / (2.0f * r);
                     float t;
 
@@ -673,74 +769,124 @@ __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool 
                     const float cs = 1.0f / sqrtf(t * t + 1.0f), sn = t * cs;
 
                     /* new col p = cs p - sn sg q, new col q = sn p + cs sg q */
-                    jr = make_float4(cs, sn, -sn * sg, cs * sg);
-                    d = make_float2(app - t * r, aqq + t * r);
+                    jr = lp ? make_float4(cs, sn, -sn * sg, cs * sg)
+                            : make_float4(cs * sg, -sn * sg, sn, cs);
+                    dpp_ = app - t * r;
+                    dqq_ = aqq + t * r;
+                    rot = true;
                 }
                 em.jac.rot[lg] = jr;
-                em.jac.dd[lg] = d;
-                em.jac.pq[lg] = make_int4(p, q, r != 0.0f, 0);
+            }
+            /* A J and V J on the own columns */
+#pragma unroll
+            for (int r = 0; r < NE; r++) {
+                const float x = cL[r], y = cR[r];
+
+                cL[r] = fmaf(jr.x, x, jr.z * y);
+                cR[r] = fmaf(jr.y, x, jr.w * y);
+            }
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                const float x = vL[r], y = vR[r];
+
+                vL[r] = fmaf(jr.x, x, jr.z * y);
+                vR[r] = fmaf(jr.y, x, jr.w * y);
             }
             wave_sync();
-            /* A <- J^T A J, one upper 2 x 2 block (i, k) at a time. */
+            /* J^T A: rows (k, NE-1-k) of every pair k */
+#pragma unroll
+            for (int k = 0; k < NP; k++) {
+                const float4 jk = em.jac.rot[k];
+                const int u = k, w = NE - 1 - k;
+                const float xl = cL[u], yl = cL[w], xr = cR[u], yr = cR[w];
+
+                cL[u] = fmaf(jk.x, xl, jk.z * yl);
+                cL[w] = fmaf(jk.y, xl, jk.w * yl);
+                cR[u] = fmaf(jk.x, xr, jk.z * yr);
+                cR[w] = fmaf(jk.y, xr, jk.w * yr);
+            }
+            /* the own 2 x 2 block exactly, as the reference sets it */
+            if (rot) {
+                const float dl = lp ? dpp_ : dqq_, dr = lp ? dqq_ : dpp_;
+
+#pragma unroll
+                for (int r = 0; r < NE; r++) {
+                    cL[r] = r == uL ? dl : (r == uR ? 0.0f : cL[r]);
+                    cR[r] = r == uR ? dr : (r == uL ? 0.0f : cR[r]);
+                }
+            }
+            wave_sync();
+            /* Next round: position u -> u + 1 on 1 .. NE-1 (NE-1 -> 1),
+               position 0 fixed; rows of A follow inside each column, and
+               the columns move between lanes: L of lane i -> L of lane
+               i+1 (lane NP-1: its own R); R of lane i -> R of lane i-1
+               (lane 0: L of lane 1). */
+#pragma unroll
+            for (int X = 0; X < 2; X++) {
+                float(&c)[NE] = X ? cR : cL;
+                const float last = c[NE - 1];
+
+#pragma unroll
+                for (int r = NE - 1; r >= 2; r--)
+                    c[r] = c[r - 1];
+                if (NE > 1)
+                    c[1] = last;
+            }
+            auto shr = [](float x) {
+                return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                    0, __builtin_bit_cast(int, x), 0x138, 0xF, 0xF, false));
+            };
+            auto shl = [](float x) {
+                return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                    0, __builtin_bit_cast(int, x), 0x130, 0xF, 0xF, false));
+            };
+            auto shri = [](int x) {
+                return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false);
+            };
+            auto shli = [](int x) {
+                return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, false);
+            };
+#pragma unroll
+            for (int r = 0; r < NE; r++) {
+                const float fromL = shr(cL[r]), fromR = shr(cR[r]), nextR = shl(cR[r]);
+                const float l = cL[r];
+
+                cL[r] = lg == 0 ? l : (lg == 1 ? fromR : fromL);
+                cR[r] = lg == NP - 1 ? l : nextR;
+            }
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                const float fromL = shr(vL[r]), fromR = shr(vR[r]), nextR = shl(vR[r]);
+                const float l = vL[r];
+
+                vL[r] = lg == 0 ? l : (lg == 1 ? fromR : fromL);
+                vR[r] = lg == NP - 1 ? l : nextR;
+            }
             {
-                int i = 0, rem = lg;
+                const int fromL = shri(iL), fromR = shri(iR), nextR = shli(iR);
+                const int l = iL;
 
-                while (i < NP && rem >= NP - i) {
-                    rem -= NP - i;
-                    i++;
-                }
-                for (int b = lg; b < NB; b += G) {
-                    const int k = i + rem;
-                    const int4 pi = em.jac.pq[i], pk = em.jac.pq[k];
-
-                    if (i == k) {
-                        if (pi.z) {
-                            const float2 d = em.jac.dd[i];
-
-                            a[pi.x * NE + pi.x] = d.x;
-                            a[pi.y * NE + pi.y] = d.y;
-                            a[pi.x * NE + pi.y] = 0.0f;
-                            a[pi.y * NE + pi.x] = 0.0f;
-                        }
-                    }
-                    else if (pi.z || pk.z) {
-                        const float4 ri = em.jac.rot[i], rk = em.jac.rot[k];
-                        const float m00 = a[pi.x * NE + pk.x], m01 = a[pi.x * NE + pk.y];
-                        const float m10 = a[pi.y * NE + pk.x], m11 = a[pi.y * NE + pk.y];
-                        /* columns (pk, qk) times J_k, then rows (pi, qi) by J_i^T */
-                        const float r00 = m00 * rk.x + m01 * rk.z, r01 = m00 * rk.y + m01 * rk.w;
-                        const float r10 = m10 * rk.x + m11 * rk.z, r11 = m10 * rk.y + m11 * rk.w;
-                        const float n00 = ri.x * r00 + ri.z * r10, n01 = ri.x * r01 + ri.z * r11;
-                        const float n10 = ri.y * r00 + ri.w * r10, n11 = ri.y * r01 + ri.w * r11;
-
-                        a[pi.x * NE + pk.x] = a[pk.x * NE + pi.x] = n00;
-                        a[pi.x * NE + pk.y] = a[pk.y * NE + pi.x] = n01;
-                        a[pi.y * NE + pk.x] = a[pk.x * NE + pi.y] = n10;
-                        a[pi.y * NE + pk.y] = a[pk.y * NE + pi.y] = n11;
-                    }
-                    rem += G;
-                    while (i < NP && rem >= NP - i) {
-                        rem -= NP - i;
-                        i++;
-                    }
-                }
+                iL = lg == 0 ? l : (lg == 1 ? fromR : fromL);
+                iR = lg == NP - 1 ? l : nextR;
             }
-            /* V <- V J on the real rows. */
-            for (int e = lg; e < N * NP; e += G) {
-                const int r = e / NP, i = e - (e / NP) * NP;
-                const int4 pi = em.jac.pq[i];
-
-                if (pi.z) {
-                    const float4 ri = em.jac.rot[i];
-                    const float vp = v[r * NE + pi.x], vq = v[r * NE + pi.y];
-
-                    v[r * NE + pi.x] = vp * ri.x + vq * ri.z;
-                    v[r * NE + pi.y] = vp * ri.y + vq * ri.w;
-                }
-            }
-            wave_sync();
         }
     }
+    /* eigenvalues and eigenvectors to LDS, by matrix index */
+    if (act) {
+        if (iL < N) {
+            a[iL * NE + iL] = at(cL, uL);
+#pragma unroll
+            for (int r = 0; r < N; r++)
+                v[r * NE + iL] = vL[r];
+        }
+        if (iR < N) {
+            a[iR * NE + iR] = at(cR, uR);
+#pragma unroll
+            for (int r = 0; r < N; r++)
+                v[r * NE + iR] = vR[r];
+        }
+    }
+    wave_sync();
 
     /* inv = V diag(1 / lambda) V^T with eigenvalues below 1e-6 of the
        largest magnitude dropped (tcoh.cl:669-677). */
@@ -755,37 +901,35 @@ __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool 
         em.jac.il[k] = fabsf(lk) < lmax ? 0.0f : 1.0f / lk;
     }
     wave_sync();
-    {
-        float w[CPL][N];
+    /* B = V diag(il) into the dead matrix area, then inv = B V^T row by
+       row, the sum over k rolled (only fallback pixels come here). */
+    for (int i = lg; i < N * N; i += G) {
+        const int r = i / N, k = i - (i / N) * N;
+
+        a[r * NE + k] = v[r * NE + k] * em.jac.il[k];
+    }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < N; r++) {
+        float sm[CPL];
 
 #pragma unroll
-        for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
+        for (int m = 0; m < CPL; m++)
+            sm[m] = 0.0f;
+#pragma unroll 2
+        for (int k = 0; k < N; k++) {
+            const float x = v[r * NE + k];
 
 #pragma unroll
-            for (int k = 0; k < N; k++)
-                w[m][k] = c < N ? v[c * NE + k] * em.jac.il[k] : 0.0f;
-        }
-#pragma unroll
-        for (int r = 0; r < N; r++) {
-            float sm[CPL];
+            for (int m = 0; m < CPL; m++) {
+                const int c = lg + m * G;
 
-#pragma unroll
-            for (int m = 0; m < CPL; m++)
-                sm[m] = 0.0f;
-#pragma unroll
-            for (int k = 0; k < N; k++) {
-                const float x = v[r * NE + k];
-
-#pragma unroll
-                for (int m = 0; m < CPL; m++)
-                    sm[m] = fmaf(x, w[m][k], sm[m]);
+                sm[m] = fmaf(x, c < N ? a[c * NE + k] : 0.0f, sm[m]);
             }
-#pragma unroll
-            for (int m = 0; m < CPL; m++)
-                inv[m][r] = sm[m];
-            TCOH_STEP_FENCE();
         }
+#pragma unroll
+        for (int m = 0; m < CPL; m++)
+            inv[m][r] = sm[m];
     }
     wave_sync();
 }

@@ -34,8 +34,10 @@ template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
     return ((cov > S::EIG_BYTES ? cov : S::EIG_BYTES) + 15) & ~15;
 }
 
-/* COV: 0 plain FP32, 1 split FP16 MFMA, 2 split BF16 MFMA. */
-template <int N, int COV, int EMI>
+/* cov: 0 plain FP32, 1 split FP16 MFMA, 2 split BF16 MFMA (uniform per
+   launch: one code object serves the three forms, so the linking code is
+   compiled once per depth and estimator). */
+template <int N, int EMI>
 __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
                                                const uint2 *__restrict__ hl,
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
                                                float *__restrict__ coh,
                                                float *__restrict__ pairs,
                                                int row0, int nrows,
-                                               int scratch_bytes)
+                                               int scratch_bytes, int cov)
 {
     typedef LinkShape<N> S;
     constexpr int G = S::G, PPW = S::PPW, NT = S::NT, NB = S::NB;
@@ -63,13 +65,16 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
     const int cnt = live ? count[p] : -1;
     const bool est = live && cnt >= kp.min_looks;
 
-    if (COV == 0)
+    if (cov == 0)
         cov_plain_group<N, G>(kp, slc, mask, p, cnt, est, live, lg,
                               reinterpret_cast<float *>(scratch) + gi * N,
                               tl + gi * NT, pairs);
+    else if (cov == 1)
+        cov_mfma_band<N, NB, PPW, 0>(kp, hl, mask, count, row0 + r, tx * PPW,
+                                     scratch, tl, pairs);
     else
-        cov_mfma_band<N, NB, PPW, COV == 2>(kp, hl, mask, count, row0 + r,
-                                            tx * PPW, scratch, tl, pairs);
+        cov_mfma_band<N, NB, PPW, 1>(kp, hl, mask, count, row0 + r, tx * PPW,
+                                     scratch, tl, pairs);
     wave_sync();
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(scratch);
     const float gam = link_group<N, G, EMI>(kp, tl + gi * NT, est, lg, sh[gi],
@@ -90,35 +95,23 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
     const int scratch = link_scratch_bytes<N>(kp, precision);
     const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::EMI_BYTES : 0);
 
-#define TCOH_LINK_LAUNCH(COV, EMI)                                              \
+#define TCOH_LINK_LAUNCH(EMI)                                                   \
     do {                                                                        \
         if (lds > 65536) {                                                      \
             const hipError_t e = hipFuncSetAttribute(                           \
-                reinterpret_cast<const void *>(&k_link<N, COV, EMI>),           \
+                reinterpret_cast<const void *>(&k_link<N, EMI>),                \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);          \
             if (e != hipSuccess)                                                \
                 return e;                                                       \
         }                                                                       \
-        hipLaunchKernelGGL((k_link<N, COV, EMI>), grid, dim3(WAVE), lds, s, kp, \
-                           slc, hl, mask, count, coh, pairs, row0, nrows,       \
-                           scratch);                                            \
+        hipLaunchKernelGGL((k_link<N, EMI>), grid, dim3(WAVE), lds, s, kp, slc, \
+                           hl, mask, count, coh, pairs, row0, nrows, scratch,   \
+                           precision);                                          \
     } while (0)
-    if (kp.emi) {
-        if (precision == 0)
-            TCOH_LINK_LAUNCH(0, 1);
-        else if (precision == 1)
-            TCOH_LINK_LAUNCH(1, 1);
-        else
-            TCOH_LINK_LAUNCH(2, 1);
-    }
-    else {
-        if (precision == 0)
-            TCOH_LINK_LAUNCH(0, 0);
-        else if (precision == 1)
-            TCOH_LINK_LAUNCH(1, 0);
-        else
-            TCOH_LINK_LAUNCH(2, 0);
-    }
+    if (kp.emi)
+        TCOH_LINK_LAUNCH(1);
+    else
+        TCOH_LINK_LAUNCH(0);
 #undef TCOH_LINK_LAUNCH
     return hipGetLastError();
 }

@@ -79,7 +79,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     float2 z[CPL];
 
     if (!EMI) {
-        hermitian_eigenvector<N, G, CPL>(A, sh, N - 1, lg, z);
+        hermitian_eigenvector<N, G, CPL, false>(A, sh, N - 1, lg, z);
     }
     else {
         float inv[CPL][N];
@@ -100,7 +100,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
                 A[m][r] = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
             }
         }
-        hermitian_eigenvector<N, G, CPL>(A, sh, 0, lg, z);
+        hermitian_eigenvector<N, G, CPL, true>(A, sh, 0, lg, z);
     }
 
     /* Phasors relative to date N/2 or to the last strongest component
@@ -140,8 +140,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     }
     const float2 zref = sh.z[ref];
     const float refm = hypotf(zref.x, zref.y);
-    const float rrm = rcpf(refm);
-    const float refr = zref.x * rrm, refi = -zref.y * rrm;
+    const float refr = qdiv<EMI>(zref.x, refm), refi = qdiv<EMI>(-zref.y, refm);
     const float floor_ = lmax * PHASOR_FLOOR;
     wave_sync();
 #pragma unroll
@@ -150,9 +149,8 @@ __device__ __forceinline__ float link_group(const KParams &kp,
         const float re = z[m].x * refr - z[m].y * refi;
         const float im = z[m].x * refi + z[m].y * refr;
         const float mg = sqrtf(fmaf(re, re, im * im));
-        const float rmg = rcpf(mg);
-
-        z[m] = mg > floor_ ? make_float2(re * rmg, im * rmg) : make_float2(1.f, 0.f);
+        z[m] = mg > floor_ ? make_float2(qdiv<EMI>(re, mg), qdiv<EMI>(im, mg))
+                           : make_float2(1.f, 0.f);
         if (c < N)
             sh.z[c] = z[m];
     }

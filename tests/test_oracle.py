@@ -323,3 +323,26 @@ def test_nulls_and_zero_samples(cref, tcoh):
     assert np.isnan(coh[10:13, 4:9]).all() and (count[10:13, 4:9] == -1).all()
     assert np.isnan(coh[20, 20]) and count[20, 20] == -1
     assert np.isfinite(coh[25, 5])
+
+
+def test_emi_fallback_scene_reaches_the_jacobi_path(cref, fp64, tcoh):
+    """The GPU test scene of the EMI fallback really holds pixels whose |T|
+    fails the Cholesky floor (so the Jacobi pseudo-inverse is exercised),
+    and others that pass it."""
+    reg, params = tcoh.synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
+                                            gamma0=(0.6, 0.99))
+    stack = tcoh.synth.scene_stack(30, 40, 40, reg, params, seed=0)
+    slc, amp, valid, prow, pcols = strip_of(stack, (21, 7))
+    s = cref.settings(30, shp_test="ad", min_shp=20, estimator="emi", bias=True,
+                      pairs="all")
+    _c, count, _p, mask, _f = cref.run(s, pcols, slc, amp, valid, prow, 40, 40)
+    est = np.flatnonzero(count >= s.min_looks)
+    y, x = est // 40, est % 40
+    v = np.asarray(slc, np.float32).reshape(-1, 30, 2)
+    sc = v[..., 0].astype(np.float64) + 1j * v[..., 1]
+    c = fp64.covariance(sc, fp64.mask_bits(mask[est], 147), y * pcols + x,
+                        fp64.window_offsets(21, 7, pcols))
+    t = fp64.coherence(c, count[est], True)
+    with np.errstate(all="ignore"):
+        _piv, ok = fp64.cholesky_pivots(np.abs(t))
+    assert 0.05 < np.mean(~ok) < 0.6, np.mean(~ok)
