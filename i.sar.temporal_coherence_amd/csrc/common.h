@@ -190,13 +190,81 @@ template <int G> __device__ __forceinline__ int gmaxi(int v)
     return v;
 }
 
-template <int G> __device__ __forceinline__ unsigned long long
-gor64(unsigned long long v)
+template <int G> __device__ __forceinline__ int gori(int v)
 {
+    if (G <= 16) {
+        if (G >= 2)
+            v |= dpp_i<DPP_XOR1>(v);
+        if (G >= 4)
+            v |= dpp_i<DPP_XOR2>(v);
+        if (G >= 8)
+            v |= dpp_i<DPP_HALF_MIRROR>(v);
+        if (G >= 16)
+            v |= dpp_i<DPP_MIRROR>(v);
+        return v;
+    }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1)
         v |= __shfl_xor(v, o, G);
     return v;
+}
+
+template <int G> __device__ __forceinline__ unsigned long long
+gor64(unsigned long long v)
+{
+    const unsigned int lo = (unsigned int)gori<G>((int)(unsigned int)v);
+    const unsigned int hi = (unsigned int)gori<G>((int)(unsigned int)(v >> 32));
+
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+/* The value x of lane l (0 <= l < G, a constant once the caller's loops
+   are unrolled) of the caller's group, in every lane of the group, without
+   LDS: G = 16 is one DPP row and row_newbcast:l (DPP control 0x150 + l)
+   broadcasts its lane l; G = 8 broadcasts lanes l and 8 + l of the row and
+   each half-row keeps its own; G = 64 reads the lane into a scalar
+   register. All lanes of the wavefront must be active. */
+template <int CTRL> __device__ __forceinline__ float dpp_mov(float v)
+{
+    /* every lane of the row gets a value (no old operand to keep) */
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF,
+                                        0xF, true));
+}
+
+#define TCOH_ROW_BCAST(L)                                                      \
+    case L:                                                                    \
+        return dpp_mov<0x150 + L>(x)
+
+__device__ __forceinline__ float row_bcast(float x, int l)
+{
+    switch (l) {
+        TCOH_ROW_BCAST(0); TCOH_ROW_BCAST(1); TCOH_ROW_BCAST(2); TCOH_ROW_BCAST(3);
+        TCOH_ROW_BCAST(4); TCOH_ROW_BCAST(5); TCOH_ROW_BCAST(6); TCOH_ROW_BCAST(7);
+        TCOH_ROW_BCAST(8); TCOH_ROW_BCAST(9); TCOH_ROW_BCAST(10); TCOH_ROW_BCAST(11);
+        TCOH_ROW_BCAST(12); TCOH_ROW_BCAST(13); TCOH_ROW_BCAST(14);
+    default:
+        return dpp_mov<0x150 + 15>(x);
+    }
+}
+#undef TCOH_ROW_BCAST
+
+template <int G> __device__ __forceinline__ float gbcast(float x, int l)
+{
+    static_assert(G == 8 || G == 16 || G == 64, "group of 8, 16 or 64 lanes");
+    if (G == 16)
+        return row_bcast(x, l);
+    if (G == 8) {
+        const float a = row_bcast(x, l), b = row_bcast(x, 8 + l);
+
+        return (threadIdx.x & 8) ? b : a;
+    }
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
+template <int G> __device__ __forceinline__ float2 gbcast2(float2 x, int l)
+{
+    return make_float2(gbcast<G>(x.x, l), gbcast<G>(x.y, l));
 }
 
 /* XCD-aware tiling of a band of R rows x nbx tiles (a tile = TW

@@ -49,22 +49,15 @@
         __builtin_amdgcn_sched_barrier(0);                                     \
     } while (0)
 
-/* vw[r] = (v_r, w_r) of the current Householder step, read as one
-   16-byte word by the update. vw is dead once the reduction ends, and the
-   inverse iteration's LU factors (u0 holds 1/pivot) and result y take its
-   place. */
+/* Per-pixel LDS of the eigen solve: the tridiagonal (d, e) and tau of the
+   reduction, the inverse iteration's LU factors (u0 holds 1/pivot) and
+   result y, the phasors z. */
 template <int N> struct EigLds {
-    union {
-        float4 vw[N];
-        struct {
-            float u0[N], u1[N], u2[N], l[N], y[N];
-            int piv[N];
-        };
-    };
+    float u0[N], u1[N], u2[N], l[N], y[N];
+    int piv[N];
     float2 tau[N];
     float d[N], e[N];
     float2 z[N];
-    float2 alpha;
 };
 
 /* Cholesky factor (FP64, lower, column-major packed, with the reciprocal
@@ -99,6 +92,82 @@ template <int N> __device__ __forceinline__ int lpos(int i, int j)
     return j * N - j * (j - 1) / 2 + (i - j);
 }
 
+/* One row of the Householder update or of its matrix-vector product for
+   G = 16, with the row operand broadcast inside the multiply-adds: VOP2
+   v_fmac_f32 with DPP row_newbcast:L reads lane L of the row as src0.
+   The leading s_nop 1 gives the two wait states a DPP read needs after a
+   VALU write of its source, and one a VALU read needs after a
+   transcendental write; nothing inside the block writes a broadcast
+   source. */
+#define TCOH_FMAC_BC(D, S, B, L)                                               \
+    "v_fmac_f32_dpp " D ", " S ", " B " row_newbcast:" L                        \
+    " row_mask:0xf bank_mask:0xf\n"
+
+template <int L> struct RowOps {
+    /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c) for one column slot:
+       (ax, ay) += vx*nwx + vy*nwy + wx*nvx + wy*nvy,
+                   vy*nwx + vx*wcy + wy*nvx + wx*vcy */
+    static __device__ __forceinline__ void update1(float &ax, float &ay, float vx,
+                                                   float vy, float wx, float wy,
+                                                   float nwx, float nwy, float nvx,
+                                                   float nvy, float wcy, float vcy)
+    {
+        asm volatile("s_nop 1\n"
+                     TCOH_FMAC_BC("%0", "%2", "%6", "%12")
+                     TCOH_FMAC_BC("%0", "%3", "%7", "%12")
+                     TCOH_FMAC_BC("%0", "%4", "%8", "%12")
+                     TCOH_FMAC_BC("%0", "%5", "%9", "%12")
+                     TCOH_FMAC_BC("%1", "%3", "%6", "%12")
+                     TCOH_FMAC_BC("%1", "%2", "%10", "%12")
+                     TCOH_FMAC_BC("%1", "%5", "%8", "%12")
+                     TCOH_FMAC_BC("%1", "%4", "%11", "%12")
+                     : "+v"(ax), "+v"(ay)
+                     : "v"(vx), "v"(vy), "v"(wx), "v"(wy), "v"(nwx), "v"(nwy),
+                       "v"(nvx), "v"(nvy), "v"(wcy), "v"(vcy), "n"(L));
+    }
+    /* w partial sums: sr += vx*ax + vy*ay, sia += vy*ax, sib += vx*ay */
+    static __device__ __forceinline__ void matvec1(float &sr, float &sia, float &sib,
+                                                   float vx, float vy, float ax,
+                                                   float ay)
+    {
+        asm volatile("s_nop 1\n"
+                     TCOH_FMAC_BC("%0", "%3", "%5", "%7")
+                     TCOH_FMAC_BC("%0", "%4", "%6", "%7")
+                     TCOH_FMAC_BC("%1", "%4", "%5", "%7")
+                     TCOH_FMAC_BC("%2", "%3", "%6", "%7")
+                     : "+v"(sr), "+v"(sia), "+v"(sib)
+                     : "v"(vx), "v"(vy), "v"(ax), "v"(ay), "n"(L));
+    }
+};
+
+#define TCOH_ROW_CASES(F)                                                      \
+    F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13)  \
+    F(14) F(15)
+
+__device__ __forceinline__ void row_update1(int l, float &ax, float &ay, float2 v,
+                                            float2 w, float2 nw, float2 nv,
+                                            float wcy, float vcy)
+{
+#define TCOH_CASE(L)                                                           \
+    case L:                                                                    \
+        RowOps<L>::update1(ax, ay, v.x, v.y, w.x, w.y, nw.x, nw.y, nv.x, nv.y, \
+                           wcy, vcy);                                          \
+        break;
+    switch (l) { TCOH_ROW_CASES(TCOH_CASE) }
+#undef TCOH_CASE
+}
+
+__device__ __forceinline__ void row_matvec1(int l, float &sr, float &sia, float &sib,
+                                            float2 v, float2 a)
+{
+#define TCOH_CASE(L)                                                           \
+    case L:                                                                    \
+        RowOps<L>::matvec1(sr, sia, sib, v.x, v.y, a.x, a.y);                  \
+        break;
+    switch (l) { TCOH_ROW_CASES(TCOH_CASE) }
+#undef TCOH_CASE
+}
+
 /* Householder reduction of the group's matrix (columns in A) to the real
    tridiagonal (d, e) in LDS; the Householder vector of step k is left in
    row k of the columns c > k (A[m][k]), tau_k in LDS.
@@ -107,7 +176,12 @@ template <int N> __device__ __forceinline__ int lpos(int i, int j)
    finished at step k is skipped at compile time. Finished columns inside
    a live slot (c <= k) are updated like the others: their rows below k
    are never read again (this layout keeps Householder vectors in rows,
-   not columns), so only the reductions and LDS writes are masked. */
+   not columns), so only the reductions are masked.
+
+   Row-wise operands (alpha, v_j, w_r) are the values of the lane that owns
+   column j (slot j / G, lane j % G) and reach the other lanes of the group
+   by lane broadcasts, not LDS: a step has no LDS traffic on its critical
+   path and no load latency in its inner loops. */
 template <int N, int G, int CPL, bool EXACT>
 __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                                                EigLds<N> &sh, int lg)
@@ -122,8 +196,6 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
 
             if (min(m * G + G - 1, N - 1) < k)
                 continue;
-            if (c == k + 1)
-                sh.alpha = make_float2(A[m][k].x, -A[m][k].y);
             if (c == k)
                 sh.d[k] = A[m][k].x;
             const float x2 = fmaf(A[m][k].x, A[m][k].x, A[m][k].y * A[m][k].y);
@@ -131,8 +203,9 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             part += (c >= k + 2 && c < N) ? x2 : 0.0f;
         }
         const float xn = gsum<G>(part);
-        wave_sync();
-        const float alr = sh.alpha.x, ali = sh.alpha.y;
+        /* alpha = A[k+1][k] = conj(A[k][k+1]), row k of column k + 1 */
+        const float2 ak = gbcast2<G>(A[(k + 1) / G][k], (k + 1) % G);
+        const float alr = ak.x, ali = -ak.y;
         float beta, tr_ = 0.0f, ti_ = 0.0f, sr = 0.0f, si = 0.0f;
 
         if (xn == 0.0f && ali == 0.0f)
@@ -170,43 +243,63 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             if (c >= k + 2)
                 A[m][k] = xs;
             act[m] = live ? 1.0f : 0.0f;
-            if (live) {
-                sh.vw[c].x = vc[m].x;
-                sh.vw[c].y = vc[m].y;
-            }
         }
         if (lg == 0) {
             sh.e[k] = beta;
             sh.tau[k] = make_float2(tr_, ti_);
         }
-        wave_sync();
 
-        /* w_c = tau sum_j conj(A[j][c]) v_j (row c is conj of column c). */
+        /* w_c = tau sum_j conj(A[j][c]) v_j (row c is conj of column c);
+           constant trip counts (predicated) keep every index of A static
+           once the step loop is unrolled. */
+        /* Every product is (broadcast operand) x (lane's own operand)
+           accumulated without a sign, so that the broadcast can ride on the
+           multiply-add itself; the imaginary part keeps its two sums apart. */
+        float sr2[CPL], sia[CPL], sib[CPL];
+
+#pragma unroll
+        for (int m = 0; m < CPL; m++) {
+            sr2[m] = 0.0f;
+            sia[m] = 0.0f;
+            sib[m] = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            if (j <= k)
+                continue;
+            if (G == 16) {
+#pragma unroll
+                for (int m = 0; m < CPL; m++) {
+                    if (min(m * G + G - 1, N - 1) < k + 1)
+                        continue;
+                    row_matvec1(j % G, sr2[m], sia[m], sib[m], vc[j / G], A[m][j]);
+                }
+                continue;
+            }
+            const float2 v = gbcast2<G>(vc[j / G], j % G);
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++) {
+                if (min(m * G + G - 1, N - 1) < k + 1)
+                    continue;
+                sr2[m] = fmaf(v.x, A[m][j].x, sr2[m]);
+                sr2[m] = fmaf(v.y, A[m][j].y, sr2[m]);
+                sia[m] = fmaf(v.y, A[m][j].x, sia[m]);
+                sib[m] = fmaf(v.x, A[m][j].y, sib[m]);
+            }
+        }
         float2 wc[CPL];
         float pr = 0.0f, pi = 0.0f;
 
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
-            float sr2 = 0.0f, si2 = 0.0f;
-
             wc[m] = make_float2(0.f, 0.f);
             if (min(m * G + G - 1, N - 1) < k + 1)
                 continue;
-            /* constant trip counts (predicated) keep every index of A
-               static once the step loop is unrolled */
-#pragma unroll
-            for (int j = 0; j < N; j++) {
-                if (j <= k)
-                    continue;
-                const float4 vw = sh.vw[j];
-                const float2 v = make_float2(vw.x, vw.y);
+            const float si2 = sia[m] - sib[m];
 
-                sr2 = fmaf(A[m][j].x, v.x, sr2);
-                sr2 = fmaf(A[m][j].y, v.y, sr2);
-                si2 = fmaf(A[m][j].x, v.y, si2);
-                si2 = fmaf(-A[m][j].y, v.x, si2);
-            }
-            wc[m] = make_float2(fmaf(tr_, sr2, -ti_ * si2), fmaf(tr_, si2, ti_ * sr2));
+            wc[m] = make_float2(fmaf(tr_, sr2[m], -ti_ * si2),
+                                fmaf(tr_, si2, ti_ * sr2[m]));
             pr = fmaf(act[m], fmaf(wc[m].x, vc[m].x, wc[m].y * vc[m].y), pr);
             pi = fmaf(act[m], fmaf(wc[m].x, vc[m].y, -wc[m].y * vc[m].x), pi);
         }
@@ -218,41 +311,52 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
 
 #pragma unroll
             for (int m = 0; m < CPL; m++) {
-                const int c = lg + m * G;
-
                 if (min(m * G + G - 1, N - 1) < k + 1)
                     continue;
                 wc[m].x = fmaf(hr, vc[m].x, fmaf(-hi, vc[m].y, wc[m].x));
                 wc[m].y = fmaf(hr, vc[m].y, fmaf(hi, vc[m].x, wc[m].y));
-                if (c >= k + 1 && c < N) {
-                    sh.vw[c].z = wc[m].x;
-                    sh.vw[c].w = wc[m].y;
-                }
             }
         }
-        wave_sync();
-        TCOH_STEP_FENCE();
 
-        /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r > k. */
+        /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r > k, with the signs
+           on the lane's own operands. */
+        float2 nw[CPL], nv[CPL];
+
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
-            if (min(m * G + G - 1, N - 1) < k + 1)
-                continue;
+            nw[m] = make_float2(-wc[m].x, -wc[m].y);
+            nv[m] = make_float2(-vc[m].x, -vc[m].y);
+        }
 #pragma unroll
-            for (int r = 0; r < N; r++) {
-                if (r <= k)
+        for (int r = 0; r < N; r++) {
+            if (r <= k)
+                continue;
+            if (G == 16) {
+#pragma unroll
+                for (int m = 0; m < CPL; m++) {
+                    if (min(m * G + G - 1, N - 1) < k + 1)
+                        continue;
+                    row_update1(r % G, A[m][r].x, A[m][r].y, vc[r / G], wc[r / G],
+                                nw[m], nv[m], wc[m].y, vc[m].y);
+                }
+                continue;
+            }
+            const float2 v = gbcast2<G>(vc[r / G], r % G);
+            const float2 w = gbcast2<G>(wc[r / G], r % G);
+
+#pragma unroll
+            for (int m = 0; m < CPL; m++) {
+                if (min(m * G + G - 1, N - 1) < k + 1)
                     continue;
-                const float4 vw = sh.vw[r];
-                const float2 v = make_float2(vw.x, vw.y), w = make_float2(vw.z, vw.w);
                 float ax = A[m][r].x, ay = A[m][r].y;
 
-                ax = fmaf(-v.x, wc[m].x, ax);
-                ax = fmaf(-v.y, wc[m].y, ax);
-                ax = fmaf(-w.x, vc[m].x, ax);
-                ax = fmaf(-w.y, vc[m].y, ax);
-                ay = fmaf(-v.y, wc[m].x, ay);
+                ax = fmaf(v.x, nw[m].x, ax);
+                ax = fmaf(v.y, nw[m].y, ax);
+                ax = fmaf(w.x, nv[m].x, ax);
+                ax = fmaf(w.y, nv[m].y, ax);
+                ay = fmaf(v.y, nw[m].x, ay);
                 ay = fmaf(v.x, wc[m].y, ay);
-                ay = fmaf(-w.y, vc[m].x, ay);
+                ay = fmaf(w.y, nv[m].x, ay);
                 ay = fmaf(w.x, vc[m].y, ay);
                 A[m][r] = make_float2(ax, ay);
             }
