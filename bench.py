@@ -41,22 +41,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def prepare_on_device(torch, stack, ha, hr):
-    """read_strip (main.c:52-127) on the GPU for one whole-frame strip."""
+def prepare_on_device(tcoh, torch, s, stack, ha, hr):
+    """read_strip (main.c:52-127) on the GPU for one whole-frame strip, by
+    the library's strip kernel (tcoh_prepare_strip_device) from the date
+    planes; returns the strip buffers and the kernel's device time (ms,
+    HIP events on the stream it runs on, second of two calls)."""
     n, rows, cols = stack.shape
     pr, pc = rows + 2 * ha, cols + 2 * hr
-    x = torch.zeros((pr, pc, n), dtype=torch.complex64, device=stack.device)
-    x[ha:ha + rows, hr:hr + cols] = stack.permute(1, 2, 0)
-    re, im = x.real.contiguous(), x.imag.contiguous()
-    inside = torch.zeros((pr, pc), dtype=torch.bool, device=stack.device)
-    inside[ha:ha + rows, hr:hr + cols] = True
-    valid = inside & ((re != 0) | (im != 0)).all(dim=-1)
-    amp = torch.sqrt(re * re + im * im)
-    amp = torch.sort(amp, dim=-1).values
-    vf = valid.unsqueeze(-1)
-    amp = torch.where(vf, amp, torch.zeros_like(amp)).contiguous()
-    slc = torch.view_as_real(torch.where(vf, x, torch.zeros_like(x))).contiguous()
-    return slc, amp, valid.to(torch.uint8).contiguous()
+    i = stack.real.contiguous()
+    q = stack.imag.contiguous()
+    slc = torch.empty((pr, pc, n, 2), dtype=torch.float32, device=stack.device)
+    amp = torch.empty((pr, pc, n), dtype=torch.float32, device=stack.device)
+    valid = torch.empty((pr, pc), dtype=torch.uint8, device=stack.device)
+    # a stream of its own: a null handle would mean the library handle's
+    # stream, which events recorded by torch would not bracket
+    st = torch.cuda.Stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    with tcoh.Backend(s, pc, platform=0, device=0) as be:
+        for rep in range(2):
+            e0.record(st)
+            be.prepare_strip_device(i.data_ptr(), q.data_ptr(), rows, cols, -ha, pr, hr,
+                                    slc.data_ptr(), amp.data_ptr(), valid.data_ptr(),
+                                    stream=st.cuda_stream)
+            e1.record(st)
+        torch.cuda.synchronize()
+    return slc, amp, valid, e0.elapsed_time(e1)
 
 
 def flops_eigen(n):
@@ -215,7 +226,7 @@ def main():
     reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
                                             gamma0=gamma0)
     stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
-    slc, amp, valid = prepare_on_device(torch, stack, ha, hr)
+    slc, amp, valid, prep_ms = prepare_on_device(tcoh, torch, s, stack, ha, hr)
     del stack
     pcols = cols + 2 * hr
     coh = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
@@ -286,6 +297,7 @@ def main():
                                "decorrelation scene (16 cells, gamma0 ~ U(%g, %g)) seed 0"
                                % (rows, cols, label, gamma0[0], gamma0[1]),
                    "estimated_pixels": est, "mean_shp": float(looks.mean()),
+                   "strip_prep_ms": round(prep_ms, 3),
                    "stage_ms": dict(zip(stage_names, np.round(st_ms, 3).tolist()))},
         "roofline": roofline,
     }

@@ -130,6 +130,9 @@ def lib():
     L.tcoh_last_error.restype = C.c_char_p
     L.tcoh_prepare_strip.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
                                      C.c_int, C.c_int, C.c_int, vp, vp, vp]
+    L.tcoh_prepare_strip_device.argtypes = [vp, C.c_int, C.c_int, vp, vp,
+                                            C.POINTER(Geometry), vp, C.c_int,
+                                            C.c_int, C.c_int, vp, vp, vp, vp]
     L.orbit_state.argtypes = [C.POINTER(Orbit), C.c_double, vp, vp]
     L.tcoh_geometry_init.argtypes = [C.POINTER(Geometry), C.POINTER(Scene),
                                      C.c_int, C.c_int, C.POINTER(Orbit),
@@ -137,6 +140,13 @@ def lib():
     L.geometry_phases.argtypes = [C.POINTER(Geometry), C.c_int, C.c_int,
                                   C.c_double, vp]
     L.geometry_free.argtypes = [C.POINTER(Geometry)]
+    # The HIP runtime the library links (found through the library's own
+    # dependencies), for DeviceBuffer.
+    L.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+    L.hipFree.argtypes = [vp]
+    L.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+    L.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
+    L.hipDeviceSynchronize.argtypes = []
     L.tcoh_set_error_mode(1)
     _lib = L
     return L
@@ -245,6 +255,18 @@ class Backend:
         """tcoh_run_device on device pointers (ints), asynchronous."""
         lib().tcoh_run_device(self.h, slc_ptr, amp_ptr, valid_ptr, padded_rows,
                               rows, cols, coh_ptr, count_ptr, pairs_ptr, stream)
+        _raise_if_error()
+
+    def prepare_strip_device(self, i_ptr, q_ptr, nrows, ncols, row0, padded_rows,
+                             halo_cols, slc_ptr, amp_ptr, valid_ptr, geometry=None,
+                             heights_ptr=None, stream=None):
+        """tcoh_prepare_strip_device (read_strip, main.c:52-127, on the
+        device) on device pointers (ints); geometry: a Geometry from
+        make_geometry or None. Asynchronous."""
+        g = None if geometry is None else C.byref(geometry)
+        lib().tcoh_prepare_strip_device(self.h, nrows, ncols, i_ptr, q_ptr, g,
+                                        heights_ptr, row0, padded_rows, halo_cols,
+                                        slc_ptr, amp_ptr, valid_ptr, stream)
         _raise_if_error()
 
     def close(self):
@@ -364,6 +386,66 @@ def geometry_grid(orbits, scene, ref, hmin, hmax, h0=0.0):
     heights = np.array(g.heights[:nh])
     L.geometry_free(C.byref(g))
     return phase.reshape(shape), heights
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime that libtcoh_hip.so itself uses,
+    for the device entry points (tcoh_run_device,
+    tcoh_prepare_strip_device) without a framework; ptr is the device
+    address."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes, fill=None):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        if lib().hipMalloc(C.byref(p), max(self.nbytes, 1)) != 0:
+            raise RuntimeError("hipMalloc of %d bytes failed" % self.nbytes)
+        self.ptr = p.value
+        if fill is not None:
+            lib().hipMemset(self.ptr, fill, max(self.nbytes, 1))
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if lib().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, cls.H2D) != 0:
+            raise RuntimeError("hipMemcpy to the device failed")
+        return b
+
+    def to_array(self, dtype, shape=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        lib().hipDeviceSynchronize()
+        if lib().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H) != 0:
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out if shape is None else out.reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            lib().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_geometry(phase, heights, h0=0.0, step=16):
+    """A struct geometry over a phase grid (grid_rows, grid_cols, nheights,
+    n) float64 and its heights; the grid array must outlive the struct (it
+    is kept as the struct's attribute _keep)."""
+    phase = np.ascontiguousarray(phase, np.float64)
+    gr, gcn, nh, n = phase.shape
+    g = Geometry()
+    g.ndates, g.grid_rows, g.grid_cols, g.step, g.nheights = n, gr, gcn, step, nh
+    for k in range(nh):
+        g.heights[k] = float(heights[k])
+    g.phase = phase.ctypes.data_as(C.POINTER(C.c_double))
+    g.h0 = h0
+    g._keep = phase
+    return g
 
 
 def list_devices():

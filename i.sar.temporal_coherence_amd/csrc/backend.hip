@@ -172,6 +172,8 @@ struct ocl {
     /* scratch */
     DevBuf ampT, lstat, mask, adtab, hl, absmax;
     int adtab_n;
+    /* the phase grid of tcoh_prepare_strip_device */
+    DevBuf grid;
 };
 
 static int env_precision()
@@ -547,6 +549,83 @@ extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
     });
 }
 
+extern "C" void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
+                                          const void *i, const void *q,
+                                          const struct geometry *g,
+                                          const void *heights, int row0,
+                                          int padded_rows, int halo_cols,
+                                          void *slc, void *amp, void *valid,
+                                          void *stream)
+{
+    guarded_void([&] {
+        const int n = o->s.ndates;
+        const int pc = ncols + 2 * halo_cols;
+        hipStream_t st = stream ? (hipStream_t)stream : o->stream;
+        StripArgs sa;
+
+        if (nrows <= 0 || ncols <= 0 || padded_rows < 0 || halo_cols < 0)
+            fail("Invalid strip: %d x %d region, %d padded rows, %d halo "
+                 "columns", nrows, ncols, padded_rows, halo_cols);
+        if (pc != o->padded_cols)
+            fail("Strip of %d columns and halo %d does not match the padded "
+                 "width %d", ncols, halo_cols, o->padded_cols);
+        if ((size_t)padded_rows * pc > (size_t)INT32_MAX)
+            fail("Strip of %d x %d pixels is too large", padded_rows, pc);
+        if (!i || !q || !slc || !amp || !valid)
+            fail("Strip preparation without date planes or output buffers");
+        check(hipSetDevice(o->device), "hipSetDevice");
+        sa.i = (const float *)i;
+        sa.q = (const float *)q;
+        sa.heights = (const double *)heights;
+        sa.grid = nullptr;
+        sa.slc = (float2 *)slc;
+        sa.amp = (float *)amp;
+        sa.valid = (unsigned char *)valid;
+        sa.n = n;
+        sa.nrows = nrows;
+        sa.ncols = ncols;
+        sa.row0 = row0;
+        sa.prow = padded_rows;
+        sa.halo = halo_cols;
+        sa.pc = pc;
+        sa.grid_cols = 0;
+        sa.step = 1;
+        sa.nh = 1;
+        sa.hx[0] = sa.hx[1] = sa.hx[2] = 0.0;
+        sa.h0 = 0.0;
+        if (g) {
+            /* geometry_phases reads nodes gr and gr + 1 (gc and gc + 1) of
+               every region cell (geometry.c:264-284). */
+            if (g->ndates != n)
+                fail("Phase grid of %d dates for a stack of %d", g->ndates, n);
+            if (!g->phase || g->step < 1 || (g->nheights != 1 && g->nheights != 3))
+                fail("Invalid phase grid");
+            if ((nrows - 1) / g->step + 1 >= g->grid_rows ||
+                (ncols - 1) / g->step + 1 >= g->grid_cols)
+                fail("Phase grid of %d x %d nodes does not cover a %d x %d "
+                     "region", g->grid_rows, g->grid_cols, nrows, ncols);
+            const size_t bytes = (size_t)g->grid_rows * g->grid_cols *
+                                 g->nheights * n * sizeof(double);
+
+            o->grid.ensure(bytes, "hipMalloc(phase grid)");
+            check(hipMemcpyAsync(o->grid.p, g->phase, bytes,
+                                 hipMemcpyHostToDevice, st),
+                  "hipMemcpy(phase grid)");
+            sa.grid = o->grid.as<double>();
+            sa.grid_cols = g->grid_cols;
+            sa.step = g->step;
+            sa.nh = g->nheights;
+            for (int k = 0; k < 3; k++)
+                sa.hx[k] = g->heights[k];
+            sa.h0 = g->h0;
+        }
+        check(launch_strip(sa, st), "strip kernel");
+        if (g)
+            /* the grid buffer is reused by the next call */
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
 extern "C" void ocl_close(struct ocl *o)
 {
     if (!o)
@@ -557,7 +636,7 @@ extern "C" void ocl_close(struct ocl *o)
         DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
                           &o->count, &o->pairs, &o->ampT,  &o->lstat,
                           &o->mask, &o->adtab, &o->hl,
-                          &o->absmax};
+                          &o->absmax, &o->grid};
         for (DevBuf *b : bufs)
             b->release();
         o->timer.release();

@@ -333,6 +333,20 @@ hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
                         int bf16, float scale, hipStream_t s);
 hipError_t launch_absmax(const float2 *slc, size_t n, unsigned int *out,
                          hipStream_t s);
+/* Strip preparation (main.c:52-127) on the device. */
+struct StripArgs {
+    const float *i, *q;       /* date planes [n][nrows][ncols], NaN null */
+    const double *heights;    /* [nrows][ncols], NaN null; or none */
+    const double *grid;       /* phase grid [node][height][date]; or none */
+    float2 *slc;              /* [padded pixel][n] */
+    float *amp;               /* [padded pixel][n], increasing */
+    unsigned char *valid;     /* [padded pixel] */
+    int n, nrows, ncols;      /* dates, region size */
+    int row0, prow, halo, pc; /* first region row, padded rows, halo, width */
+    int grid_cols, step, nh;  /* phase grid */
+    double hx[3], h0;         /* grid heights, height without elevation */
+};
+hipError_t launch_strip(const StripArgs &sa, hipStream_t s);
 /* Fused covariance + linking of output rows row0 .. row0+nrows-1
    (precision: 0 plain, 1 split FP16, 2 split BF16), one instantiation
    per stack depth N. */

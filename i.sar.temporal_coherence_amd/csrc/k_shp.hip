@@ -201,15 +201,18 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                         ib += !take_a;
                         s = s + adtab[j * (n + 1) + m];
                     }
-                    ok = __fdiv_rn(__fsub_rn(__fmul_rn(kp.ad_norm, s), 1.0f),
-                                   kp.ad_sigma) <= kp.ad_crit;
+                    /* plain operators: contraction is off in this file
+                       (hipcc's __f*_rn helpers are contractable operators
+                       and a native square root) */
+                    ok = (kp.ad_norm * s - 1.0f) / kp.ad_sigma <= kp.ad_crit;
                 }
                 else {
                     const float2 st = lstat[cand];
-                    const float se = __fsqrt_rn(__fdiv_rn(var_a + st.y, (float)n));
+                    /* sqrtf and / are correctly rounded (hipcc's default
+                       -fhip-fp32-correctly-rounded-divide-sqrt) */
+                    const float se = sqrtf((var_a + st.y) / (float)n);
 
-                    ok = !(se > 0.0f) ||
-                         __fdiv_rn(fabsf(mean_a - st.x), se) <= kp.tlog_crit;
+                    ok = !(se > 0.0f) || fabsf(mean_a - st.x) / se <= kp.tlog_crit;
                 }
             }
             word |= (uint32_t)ok << bit;

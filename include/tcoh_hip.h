@@ -157,6 +157,25 @@ void tcoh_prepare_strip(int n, int nrows, int ncols, const float *i,
                         int padded_rows, int halo_cols, float *slc,
                         float *amp, unsigned char *valid);
 
+/* read_strip (main.c:52-127) on the device, for date planes already in
+   device memory: the same strip as tcoh_prepare_strip (validity and
+   amplitudes bit for bit; a rotated sample may differ by one float ulp,
+   from the device's FP64 sin/cos) into device buffers slc, amp, valid with
+   ocl_run's layouts, for the handle's stack depth and padded width
+   (ncols + 2 halo_cols). i, q: float[n][nrows][ncols] (NaN is a null).
+   g (host struct, may be NULL): the flat-earth and topographic phase of
+   geometry_phases (geometry.c:256-291) is removed from every date, at the
+   height heights[row][col] (double[nrows][ncols] in device memory, NaN a
+   null that makes the pixel invalid) or, with heights NULL, at g->h0.
+   Enqueued on stream (NULL: the handle's stream); returns without waiting,
+   except after copying g's grid to the device. Extension: the reference
+   prepares strips on the host. */
+void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
+                               const void *i, const void *q,
+                               const struct geometry *g, const void *heights,
+                               int row0, int padded_rows, int halo_cols,
+                               void *slc, void *amp, void *valid, void *stream);
+
 /* ocl_run on device-resident buffers (hipMalloc'd, same layouts), enqueued
    on stream (a hipStream_t; NULL is the handle's own stream). Returns
    without synchronising. */

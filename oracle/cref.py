@@ -153,6 +153,27 @@ def prepare_strip(i, q, row0, padded_rows, halo_cols, phase=None, elev_null=None
     return slc, amp, valid
 
 
+def pixel_phases(grid, heights, nrows, ncols, elevation=None, h0=0.0):
+    """geometry_phases (geometry.c:256-291) of every region cell as planes
+    (n, nrows, ncols) float64: the FP64 restatement at the cell's height
+    (elevation[row, col]; NaN heights give 0) or at h0."""
+    grid = np.ascontiguousarray(grid, np.float64)
+    gr, gcn, nh, n = grid.shape
+    hs = np.zeros(3)
+    hs[:nh] = heights
+    out = np.zeros((n, nrows, ncols))
+    buf = np.zeros(n)
+    flat = grid.reshape(-1)
+    for r in range(nrows):
+        for c in range(ncols):
+            h = h0 if elevation is None else float(elevation[r, c])
+            if np.isnan(h):
+                continue
+            lib().oracle_geometry_phases(n, gcn, nh, hs, flat, r, c, h, buf)
+            out[:, r, c] = buf
+    return out
+
+
 def geometry_grid(orbits, scene, ref, hmin, hmax):
     """geometry.c:161-254 in FP64 on the CPU: the phase grid (grid_rows,
     grid_cols, nheights, n) and heights; orbits and scene as

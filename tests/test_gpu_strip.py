@@ -1,0 +1,147 @@
+"""Strip preparation on the device (tcoh_prepare_strip_device) against the
+oracle's restatement of read_strip (main.c:52-127), through the C ABI.
+
+Validity and amplitudes must match bit for bit, and so must the samples
+without a geometric phase. With one, the samples are rotated by the same
+FP64 phase (the restatement of geometry.c:256-291 evaluated in the same
+order) through the device's FP64 sin/cos instead of the host's: both are
+within an ulp of the exact value, so a rotated sample may differ by one
+float ulp; asserted: at most one ulp, and nearly all identical.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WINDOW = (9, 5)
+
+
+def planes(tcoh, n, rows, cols, seed):
+    """I/Q date planes with nulls and zero samples planted."""
+    stack = tcoh.synth.ds_stack(n, 0.6, rows=rows, cols=cols, seed=seed)
+    i = np.ascontiguousarray(stack.real, np.float32)
+    q = np.ascontiguousarray(stack.imag, np.float32)
+    rng = np.random.default_rng(seed + 100)
+    for _ in range(12):
+        k, r, c = rng.integers(n), rng.integers(rows), rng.integers(cols)
+        (i if rng.integers(2) else q)[k, r, c] = np.nan
+    for _ in range(6):
+        k, r, c = rng.integers(n), rng.integers(rows), rng.integers(cols)
+        i[k, r, c] = q[k, r, c] = 0.0
+    i[1, 2, 3] = 0.0  # one zero component alone keeps the pixel valid
+    return i, q
+
+
+def prepare_on_device(tcoh, i, q, row0, prow, halo, geometry=None, heights=None):
+    D = tcoh.api.DeviceBuffer
+    n, rows, cols = i.shape
+    pc = cols + 2 * halo
+    s = tcoh.make_settings(n, window=(WINDOW[0], 2 * halo + 1), min_shp=3)
+    di, dq = D.from_array(i), D.from_array(q)
+    dh = None if heights is None else D.from_array(np.asarray(heights, np.float64))
+    # outputs start as garbage (0x5a bytes): every byte must be written
+    slc = D(prow * pc * n * 8, fill=0x5A)
+    amp = D(prow * pc * n * 4, fill=0x5A)
+    valid = D(prow * pc, fill=0x5A)
+    with tcoh.Backend(s, pc, platform=0, device=0) as be:
+        be.prepare_strip_device(di.ptr, dq.ptr, rows, cols, row0, prow, halo, slc.ptr,
+                                amp.ptr, valid.ptr, geometry=geometry,
+                                heights_ptr=None if dh is None else dh.ptr)
+        out = (slc.to_array(np.float32), amp.to_array(np.float32),
+               valid.to_array(np.uint8))
+    return out
+
+
+def ulps(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    ai = np.where(ai < 0, -(ai & 0x7FFFFFFF), ai)
+    bi = np.where(bi < 0, -(bi & 0x7FFFFFFF), bi)
+    return np.abs(ai - bi)
+
+
+@pytest.mark.parametrize("n", [6, 30, 45])
+def test_strip_without_geometry_is_bit_exact(tcoh, cref, n):
+    rows, cols = 37, 45
+    ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
+    i, q = planes(tcoh, n, rows, cols, seed=n)
+    # the whole region with its halo, a strip inside it, a strip past its end
+    for row0, prow in ((-ha, rows + 2 * ha), (7, 16), (rows - 5, 12)):
+        got = prepare_on_device(tcoh, i, q, row0, prow, hr)
+        want = cref.prepare_strip(i, q, row0, prow, hr)
+        for g, w, name in zip(got, want, ("slc", "amp", "valid")):
+            assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), (name, row0)
+        assert want[2].sum() > 0
+
+
+@pytest.mark.parametrize("n,elevation", [(6, True), (30, True), (30, False)])
+def test_strip_with_geometric_phase(tcoh, cref, n, elevation):
+    rows, cols = 37, 45
+    ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
+    i, q = planes(tcoh, n, rows, cols, seed=3 * n)
+    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, seed=4)
+    grid, hs, status = cref.geometry_grid(orbits, scene, ref, 0.0, 500.0)
+    assert status == 0 and grid.shape[2] == 3
+    rng = np.random.default_rng(5)
+    heights = rng.uniform(-20.0, 520.0, (rows, cols)) if elevation else None
+    if elevation:
+        heights[rng.integers(rows, size=9), rng.integers(cols, size=9)] = np.nan
+    h0 = 123.5
+    g = tcoh.api.make_geometry(grid, hs, h0=h0)
+    row0, prow = -ha, rows + 2 * ha
+    got = prepare_on_device(tcoh, i, q, row0, prow, hr, geometry=g, heights=heights)
+    ph = cref.pixel_phases(grid, hs, rows, cols, elevation=heights, h0=h0)
+    en = None if heights is None else np.isnan(heights).astype(np.uint8)
+    want = cref.prepare_strip(i, q, row0, prow, hr, ph, en)
+    assert np.array_equal(got[2], want[2])
+    assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
+    d = ulps(got[0], want[0])
+    assert d.max() <= 1, d.max()
+    assert np.mean(d == 0) > 0.999
+    # the rotation is real: the rotated samples differ from the raw ones
+    raw = cref.prepare_strip(i, q, row0, prow, hr)
+    assert not np.array_equal(raw[0], want[0])
+
+
+def test_device_strip_feeds_the_estimate(tcoh, cref):
+    """tcoh_run_device on the device-prepared strip gives the maps of
+    ocl_run on the host-prepared strip, bit for bit."""
+    D = tcoh.api.DeviceBuffer
+    n, rows, cols = 8, 40, 33
+    ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
+    i, q = planes(tcoh, n, rows, cols, seed=21)
+    prow, pc = rows + 2 * ha, cols + 2 * hr
+    s = tcoh.make_settings(n, window=WINDOW, min_shp=9, pairs="consecutive",
+                           bias=True)
+    slc_h, amp_h, valid_h = cref.prepare_strip(i, q, -ha, prow, hr)
+    with tcoh.Backend(s, pc, platform=0, device=0) as be:
+        coh_h, count_h, pairs_h = be.run(slc_h, amp_h, valid_h, prow, rows, cols)
+        di, dq = D.from_array(i), D.from_array(q)
+        slc, amp, valid = D(prow * pc * n * 8), D(prow * pc * n * 4), D(prow * pc)
+        coh, count = D(rows * cols * 4), D(rows * cols * 4)
+        pairs = D(s.npairs * rows * cols * 4)
+        be.prepare_strip_device(di.ptr, dq.ptr, rows, cols, -ha, prow, hr, slc.ptr,
+                                amp.ptr, valid.ptr)
+        be.run_device(slc.ptr, amp.ptr, valid.ptr, prow, rows, cols, coh.ptr,
+                      count.ptr, pairs.ptr)
+        coh_d = coh.to_array(np.float32)
+        count_d = count.to_array(np.int32)
+        pairs_d = pairs.to_array(np.float32, (s.npairs, rows * cols))
+    assert np.array_equal(count_d, count_h)
+    assert np.array_equal(coh_d.view(np.uint32), coh_h.view(np.uint32))
+    assert np.array_equal(pairs_d.view(np.uint32), pairs_h.view(np.uint32))
+
+
+def test_strip_errors_are_reported(tcoh):
+    n, rows, cols = 6, 20, 20
+    s = tcoh.make_settings(n, window=WINDOW, min_shp=3)
+    buf = tcoh.api.DeviceBuffer(16384)
+    p = buf.ptr
+    with tcoh.Backend(s, cols + 4, platform=0, device=0) as be:
+        with pytest.raises(RuntimeError, match="padded width"):
+            be.prepare_strip_device(p, p, rows, cols + 1, 0, 1, 2, p, p, p)
+        grid = np.zeros((2, 2, 1, n))  # covers 16 x 16 cells, not 20 x 20
+        g = tcoh.api.make_geometry(grid, [0.0])
+        with pytest.raises(RuntimeError, match="does not cover"):
+            be.prepare_strip_device(p, p, rows, cols, 0, 1, 2, p, p, p, geometry=g)

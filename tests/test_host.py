@@ -240,3 +240,24 @@ def test_geometry_oracle_matches_exact_orbits(tcoh, cref):
     assert worst < 1e-5, worst
     # the phases vary with the baseline: a real geometric signal, not zeros
     assert np.ptp(ph[0, 0, 0]) > 100.0
+
+
+def test_pixel_phases_match_the_library(tcoh, cref):
+    """The oracle's per-cell phase planes (the checker of the device strip
+    preparation) against the library's host geometry_phases
+    (geometry.c:256-291), bit for bit, at elevation heights and at h0."""
+    rng = np.random.default_rng(8)
+    n, rows, cols = 4, 19, 35
+    grid = rng.normal(0.0, 30.0, ((rows - 1) // 16 + 2, (cols - 1) // 16 + 2, 3, n))
+    hs = np.array([-10.0, 240.0, 490.0])
+    g = tcoh.api.make_geometry(grid, hs, h0=77.0)
+    elev = rng.uniform(0.0, 500.0, (rows, cols))
+    elev[3, 4] = np.nan
+    for e in (elev, None):
+        got = cref.pixel_phases(grid, hs, rows, cols, elevation=e, h0=77.0)
+        for r, c in ((0, 0), (5, 17), (18, 34), (3, 4)):
+            h = 77.0 if e is None else e[r, c]
+            want = np.zeros(n)
+            if not np.isnan(h):
+                tcoh.api.lib().geometry_phases(C.byref(g), r, c, h, want.ctypes.data)
+            assert np.array_equal(got[:, r, c], want)
