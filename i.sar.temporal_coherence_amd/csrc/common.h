@@ -65,6 +65,19 @@ __device__ __forceinline__ float rcpf(float x)
     return __builtin_amdgcn_rcpf(x);
 }
 
+/* Hardware square root (1 ulp, one instruction), where the result only
+   has to meet a tolerance; sqrtf is the correctly rounded one, a sequence
+   of some twenty instructions under hipcc's defaults. */
+__device__ __forceinline__ float sqrt_fast(float x)
+{
+    return __builtin_amdgcn_sqrtf(x);
+}
+
+template <bool EXACT> __device__ __forceinline__ float qsqrt(float x)
+{
+    return EXACT ? sqrtf(x) : sqrt_fast(x);
+}
+
 /* a / b: correctly rounded when EXACT (the reference's division), else
    through the hardware reciprocal. */
 template <bool EXACT> __device__ __forceinline__ float qdiv(float a, float b)

@@ -41,7 +41,7 @@ __device__ __forceinline__ float2 coherence_entry(float cre, float cim,
         if (bias && looks > 1) {
             const float m2 = re * re + im * im;
             const float m2c = (looks * m2 - 1.0f) / (looks - 1.0f);
-            const float f = (m2 > 0.0f && m2c > 0.0f) ? sqrtf(m2c / m2) : 0.0f;
+            const float f = (m2 > 0.0f && m2c > 0.0f) ? sqrt_fast(m2c / m2) : 0.0f;
 
             re *= f;
             im *= f;
@@ -151,11 +151,11 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
                 continue;
             }
             const float2 t = coherence_entry(
-                cr[r][c], ci[r][c], sqrtf(diag[i]) * sqrtf(diag[j]), kp.bias, cnt);
+                cr[r][c], ci[r][c], sqrt_fast(diag[i]) * sqrt_fast(diag[j]), kp.bias, cnt);
 
             tl[upper_index(N, i, j)] = t;
             if (pidx >= 0)
-                pairs[(size_t)pidx * plane + p] = hypotf(t.x, t.y);
+                pairs[(size_t)pidx * plane + p] = sqrt_fast(t.x * t.x + t.y * t.y);
         }
     }
 }
@@ -457,7 +457,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                         const float m2 = re * re + im * im;
                         const float m2c = (fl * m2 - 1.0f) * inv_l1;
                         const float f = (m2 > 0.0f && m2c > 0.0f)
-                                            ? sqrtf(m2c) * rsqrtf(m2)
+                                            ? sqrt_fast(m2c) * rsqrtf(m2)
                                             : 0.0f;
 
                         re *= f;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
             for (int t = lane; t < kp.npairs; t += WAVE) {
                 const float2 v = tl[qq * NT + (kp.pairs_all ? t : upper_index(N, t, t + 1))];
 
-                pairs[(size_t)t * plane + p0 + qq] = sqrtf(v.x * v.x + v.y * v.y);
+                pairs[(size_t)t * plane + p0 + qq] = sqrt_fast(v.x * v.x + v.y * v.y);
             }
         }
     }

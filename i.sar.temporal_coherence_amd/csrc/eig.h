@@ -211,7 +211,7 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
         if (xn == 0.0f && ali == 0.0f)
             beta = alr;
         else {
-            beta = -copysignf(sqrtf(alr * alr + ali * ali + xn), alr);
+            beta = -copysignf(qsqrt<EXACT>(alr * alr + ali * ali + xn), alr);
             tr_ = qdiv<EXACT>(beta - alr, beta);
             ti_ = qdiv<EXACT>(-ali, beta);
             {

@@ -111,7 +111,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     for (int m = 0; m < CPL; m++) {
         const int c = lg + m * G;
 
-        mags[m] = sqrtf(fmaf(z[m].x, z[m].x, z[m].y * z[m].y));
+        mags[m] = qsqrt<EMI>(fmaf(z[m].x, z[m].x, z[m].y * z[m].y));
         if (c < N) {
             lmax = fmaxf(lmax, mags[m]);
             sh.z[c] = z[m];
@@ -123,7 +123,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     {
         const float2 zr = sh.z[N / 2];
 
-        if (sqrtf(fmaf(zr.x, zr.x, zr.y * zr.y)) <= PHASOR_FLOOR * lmax) {
+        if (qsqrt<EMI>(fmaf(zr.x, zr.x, zr.y * zr.y)) <= PHASOR_FLOOR * lmax) {
             int last = -1;
 
 #pragma unroll
@@ -139,7 +139,8 @@ __device__ __forceinline__ float link_group(const KParams &kp,
         }
     }
     const float2 zref = sh.z[ref];
-    const float refm = hypotf(zref.x, zref.y);
+    const float refm = EMI ? hypotf(zref.x, zref.y)
+                           : sqrt_fast(fmaf(zref.x, zref.x, zref.y * zref.y));
     const float refr = qdiv<EMI>(zref.x, refm), refi = qdiv<EMI>(-zref.y, refm);
     const float floor_ = lmax * PHASOR_FLOOR;
     wave_sync();
@@ -148,7 +149,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
         const int c = lg + m * G;
         const float re = z[m].x * refr - z[m].y * refi;
         const float im = z[m].x * refi + z[m].y * refr;
-        const float mg = sqrtf(fmaf(re, re, im * im));
+        const float mg = qsqrt<EMI>(fmaf(re, re, im * im));
         z[m] = mg > floor_ ? make_float2(qdiv<EMI>(re, mg), qdiv<EMI>(im, mg))
                            : make_float2(1.f, 0.f);
         if (c < N)
@@ -183,7 +184,9 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     }
     gr = gsum<G>(gr);
     gim = gsum<G>(gim);
-    return est ? hypotf(gr, gim) / (0.5f * N * (N - 1)) : __builtin_nanf("");
+    const float mod = EMI ? hypotf(gr, gim) : sqrt_fast(fmaf(gr, gr, gim * gim));
+
+    return est ? mod / (0.5f * N * (N - 1)) : __builtin_nanf("");
 }
 
 #endif
