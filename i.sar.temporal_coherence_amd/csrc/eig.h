@@ -56,7 +56,7 @@ template <int N> struct EigLds {
     float u0[N], u1[N], u2[N], l[N], y[N];
     int piv[N];
     float2 tau[N];
-    float d[N], e[N];
+    float d[N], e[N], e2[N]; /* e2 = e * e, rounded as the Sturm count's */
     float2 z[N];
 };
 
@@ -246,6 +246,7 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
         }
         if (lg == 0) {
             sh.e[k] = beta;
+            sh.e2[k] = beta * beta;
             sh.tau[k] = make_float2(tr_, ti_);
         }
 
@@ -402,8 +403,7 @@ __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
         for (int i = 1; i < N; i++) {
             if (fabsf(qv) < pivmin)
                 qv = -pivmin;
-            qv = sh.d[i] - x - sh.e[i - 1] * sh.e[i - 1] *
-                                   __builtin_amdgcn_rcpf(qv);
+            qv = sh.d[i] - x - sh.e2[i - 1] * __builtin_amdgcn_rcpf(qv);
             cnt += qv < 0.0f;
         }
         const float nlo = gmax<G>(cnt <= idx ? x : lo);
@@ -533,6 +533,10 @@ __device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
             const int c = lg + m * G;
 
             v[m] = make_float2(0.f, 0.f);
+            /* a slot whose columns are all <= k + 1 has at most v = 1 at
+               c = k + 1 (else none): known at compile time */
+            if (min(m * G + G - 1, N - 1) < k + 1)
+                continue;
             if (c == k + 1)
                 v[m] = make_float2(1.f, 0.f);
             else if (c > k + 1 && c < N)
@@ -547,6 +551,8 @@ __device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
 
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
+            if (min(m * G + G - 1, N - 1) < k + 1)
+                continue;
             const float2 vh = cmul(v[m], h);
 
             z[m].x -= vh.x;

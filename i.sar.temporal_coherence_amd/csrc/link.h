@@ -158,25 +158,34 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     wave_sync();
 
     /* gamma = |sum_{i<j} unit(T_ij) conj(z_i) z_j| / (N(N-1)/2), unit(0)
-       = 1 (tcoh.cl:726-744); lane c sums its column. */
+       = 1 (tcoh.cl:726-744). Column j has j terms; lane lg takes the
+       columns c1 = p and c2 = N - 1 - p of pairs p = lg, lg + G, ..., so
+       every lane sums N - 1 terms per pair (the middle column of an odd N
+       once). */
     float gr = 0.0f, gim = 0.0f;
+    constexpr int NPAIR = (N + 1) / 2;
 
 #pragma unroll
-    for (int m = 0; m < CPL; m++) {
-        const int c = lg + m * G;
+    for (int pp = 0; pp < (NPAIR + G - 1) / G; pp++) {
+        const int c1 = lg + pp * G, c2 = N - 1 - c1;
 
-        if (!est || m * G >= N)
+        if (!est || pp * G >= NPAIR || c1 >= NPAIR)
             continue;
+        const float2 z1 = sh.z[c1], z2 = sh.z[c2];
+
 #pragma unroll 4
-        for (int i = 0; i < N - 1; i++) {
-            const bool up = i < c && c < N;
-            const float2 t = tp[up ? upper_index(N, i, c) : 0];
-            const float m2 = fmaf(t.x, t.x, t.y * t.y);
+        for (int t = 0; t < N - 1; t++) {
+            const bool first = t < c1;
+            const int col = first ? c1 : c2;
+            const int i = first ? t : t - c1;
+            const float2 zc = first ? z1 : z2;
+            const float2 tv = tp[upper_index(N, i, col)];
+            const float m2 = fmaf(tv.x, tv.x, tv.y * tv.y);
             const float rm = __builtin_amdgcn_rsqf(m2);
-            const float ur = m2 > 0.0f ? t.x * rm : 1.0f;
-            const float ui = m2 > 0.0f ? t.y * rm : 0.0f;
-            const float2 w = cmulc(sh.z[i], z[m]);
-            const float wt = up ? 1.0f : 0.0f;
+            const float ur = m2 > 0.0f ? tv.x * rm : 1.0f;
+            const float ui = m2 > 0.0f ? tv.y * rm : 0.0f;
+            const float2 w = cmulc(sh.z[i], zc);
+            const float wt = (first || c1 != c2) ? 1.0f : 0.0f;
 
             gr = fmaf(wt, fmaf(ur, w.x, -ui * w.y), gr);
             gim = fmaf(wt, fmaf(ur, w.y, ui * w.x), gim);
