@@ -133,6 +133,8 @@ def lib():
     L.tcoh_prepare_strip_device.argtypes = [vp, C.c_int, C.c_int, vp, vp,
                                             C.POINTER(Geometry), vp, C.c_int,
                                             C.c_int, C.c_int, vp, vp, vp, vp]
+    L.tcoh_run_region.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.POINTER(Geometry),
+                                  vp, vp, vp, vp, C.c_int]
     L.orbit_state.argtypes = [C.POINTER(Orbit), C.c_double, vp, vp]
     L.tcoh_geometry_init.argtypes = [C.POINTER(Geometry), C.POINTER(Scene),
                                      C.c_int, C.c_int, C.POINTER(Orbit),
@@ -268,6 +270,26 @@ class Backend:
                                         heights_ptr, row0, padded_rows, halo_cols,
                                         slc_ptr, amp_ptr, valid_ptr, stream)
         _raise_if_error()
+
+    def run_region(self, i, q, geometry=None, heights=None, strip_rows=0):
+        """tcoh_run_region on host planes i, q (n, nrows, ncols) float32:
+        (coh, count, pairs (npairs, nrows*ncols) or None), the region
+        streamed through the device strip by strip."""
+        s = self.settings
+        i = np.ascontiguousarray(i, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        n, nrows, ncols = i.shape
+        if n != s.ndates or q.shape != i.shape:
+            raise ValueError("date planes must be (%d, rows, cols)" % s.ndates)
+        h = None if heights is None else np.ascontiguousarray(heights, np.float64)
+        coh = np.empty(nrows * ncols, np.float32)
+        count = np.empty(nrows * ncols, np.int32)
+        pairs = np.empty((s.npairs, nrows * ncols), np.float32) if s.npairs else None
+        g = None if geometry is None else C.byref(geometry)
+        lib().tcoh_run_region(self.h, nrows, ncols, _ptr(i), _ptr(q), g, _ptr(h),
+                              _ptr(coh), _ptr(count), _ptr(pairs), strip_rows)
+        _raise_if_error()
+        return coh, count, pairs
 
     def close(self):
         if self.h:

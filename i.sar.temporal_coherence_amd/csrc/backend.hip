@@ -23,6 +23,7 @@
 #include <stdexcept>
 #include <string>
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -172,8 +173,17 @@ struct ocl {
     /* scratch */
     DevBuf ampT, lstat, mask, adtab, hl, absmax;
     int adtab_n;
-    /* the phase grid of tcoh_prepare_strip_device */
+    /* the phase grid of tcoh_prepare_strip_device / tcoh_run_region */
     DevBuf grid;
+    /* tcoh_run_region: double-buffered date planes and outputs on the
+       device, pinned host staging, one strip's prepared buffers, the copy
+       stream and the events ordering the two streams */
+    DevBuf rin[2], rhgt[2], rout[2], rslc, ramp, rvalid;
+    void *hin[2] = {nullptr, nullptr}, *hout[2] = {nullptr, nullptr};
+    size_t hin_size[2] = {0, 0}, hout_size[2] = {0, 0};
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_h2d[2], ev_prep[2], ev_comp[2], ev_d2h[2];
+    bool region_ready = false;
 };
 
 static int env_precision()
@@ -549,6 +559,67 @@ extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
     });
 }
 
+/* The phase grid of g on the device, checked against the region
+   (geometry_phases reads nodes gr and gr + 1, gc and gc + 1 of every region
+   cell, geometry.c:264-284). */
+static const double *upload_grid(struct ocl *o, const struct geometry *g,
+                                 int nrows, int ncols, hipStream_t st)
+{
+    const int n = o->s.ndates;
+
+    if (g->ndates != n)
+        fail("Phase grid of %d dates for a stack of %d", g->ndates, n);
+    if (!g->phase || g->step < 1 || (g->nheights != 1 && g->nheights != 3))
+        fail("Invalid phase grid");
+    if ((nrows - 1) / g->step + 1 >= g->grid_rows ||
+        (ncols - 1) / g->step + 1 >= g->grid_cols)
+        fail("Phase grid of %d x %d nodes does not cover a %d x %d region",
+             g->grid_rows, g->grid_cols, nrows, ncols);
+    const size_t bytes =
+        (size_t)g->grid_rows * g->grid_cols * g->nheights * n * sizeof(double);
+
+    o->grid.ensure(bytes, "hipMalloc(phase grid)");
+    check(hipMemcpyAsync(o->grid.p, g->phase, bytes, hipMemcpyHostToDevice, st),
+          "hipMemcpy(phase grid)");
+    return o->grid.as<double>();
+}
+
+/* Arguments of k_strip for planes of plane_rows rows holding region rows
+   plane_row0 .. plane_row0 + plane_rows - 1; row0 is the strip's first
+   padded row in plane rows. */
+static StripArgs strip_args(const struct ocl *o, int plane_rows, int ncols,
+                            const void *i, const void *q,
+                            const struct geometry *g, const double *dgrid,
+                            const void *heights, int row0, int padded_rows,
+                            int halo_cols, int plane_row0, void *slc, void *amp,
+                            void *valid)
+{
+    StripArgs sa;
+
+    sa.i = (const float *)i;
+    sa.q = (const float *)q;
+    sa.heights = (const double *)heights;
+    sa.grid = g ? dgrid : nullptr;
+    sa.slc = (float2 *)slc;
+    sa.amp = (float *)amp;
+    sa.valid = (unsigned char *)valid;
+    sa.n = o->s.ndates;
+    sa.nrows = plane_rows;
+    sa.ncols = ncols;
+    sa.row0 = row0;
+    sa.prow = padded_rows;
+    sa.halo = halo_cols;
+    sa.pc = ncols + 2 * halo_cols;
+    sa.plane_row0 = plane_row0;
+    sa.grid_cols = g ? g->grid_cols : 0;
+    sa.step = g ? g->step : 1;
+    sa.nh = g ? g->nheights : 1;
+    for (int k = 0; k < 3; k++)
+        sa.hx[k] = g ? g->heights[k] : 0.0;
+    sa.h0 = g ? g->h0 : 0.0;
+    return sa;
+}
+
 extern "C" void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
                                           const void *i, const void *q,
                                           const struct geometry *g,
@@ -558,10 +629,8 @@ extern "C" void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
                                           void *stream)
 {
     guarded_void([&] {
-        const int n = o->s.ndates;
         const int pc = ncols + 2 * halo_cols;
         hipStream_t st = stream ? (hipStream_t)stream : o->stream;
-        StripArgs sa;
 
         if (nrows <= 0 || ncols <= 0 || padded_rows < 0 || halo_cols < 0)
             fail("Invalid strip: %d x %d region, %d padded rows, %d halo "
@@ -574,55 +643,204 @@ extern "C" void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
         if (!i || !q || !slc || !amp || !valid)
             fail("Strip preparation without date planes or output buffers");
         check(hipSetDevice(o->device), "hipSetDevice");
-        sa.i = (const float *)i;
-        sa.q = (const float *)q;
-        sa.heights = (const double *)heights;
-        sa.grid = nullptr;
-        sa.slc = (float2 *)slc;
-        sa.amp = (float *)amp;
-        sa.valid = (unsigned char *)valid;
-        sa.n = n;
-        sa.nrows = nrows;
-        sa.ncols = ncols;
-        sa.row0 = row0;
-        sa.prow = padded_rows;
-        sa.halo = halo_cols;
-        sa.pc = pc;
-        sa.grid_cols = 0;
-        sa.step = 1;
-        sa.nh = 1;
-        sa.hx[0] = sa.hx[1] = sa.hx[2] = 0.0;
-        sa.h0 = 0.0;
-        if (g) {
-            /* geometry_phases reads nodes gr and gr + 1 (gc and gc + 1) of
-               every region cell (geometry.c:264-284). */
-            if (g->ndates != n)
-                fail("Phase grid of %d dates for a stack of %d", g->ndates, n);
-            if (!g->phase || g->step < 1 || (g->nheights != 1 && g->nheights != 3))
-                fail("Invalid phase grid");
-            if ((nrows - 1) / g->step + 1 >= g->grid_rows ||
-                (ncols - 1) / g->step + 1 >= g->grid_cols)
-                fail("Phase grid of %d x %d nodes does not cover a %d x %d "
-                     "region", g->grid_rows, g->grid_cols, nrows, ncols);
-            const size_t bytes = (size_t)g->grid_rows * g->grid_cols *
-                                 g->nheights * n * sizeof(double);
+        const double *dgrid = g ? upload_grid(o, g, nrows, ncols, st) : nullptr;
+        const StripArgs sa = strip_args(o, nrows, ncols, i, q, g, dgrid, heights,
+                                        row0, padded_rows, halo_cols, 0, slc,
+                                        amp, valid);
 
-            o->grid.ensure(bytes, "hipMalloc(phase grid)");
-            check(hipMemcpyAsync(o->grid.p, g->phase, bytes,
-                                 hipMemcpyHostToDevice, st),
-                  "hipMemcpy(phase grid)");
-            sa.grid = o->grid.as<double>();
-            sa.grid_cols = g->grid_cols;
-            sa.step = g->step;
-            sa.nh = g->nheights;
-            for (int k = 0; k < 3; k++)
-                sa.hx[k] = g->heights[k];
-            sa.h0 = g->h0;
-        }
         check(launch_strip(sa, st), "strip kernel");
         if (g)
             /* the grid buffer is reused by the next call */
             check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+/* Host copies of n_parts equal-sized parts (dst_k <- src_k, bytes each)
+   on up to 8 threads: the pinned staging copies of tcoh_run_region run
+   while the GPU computes, and one thread alone would not keep up with
+   the device. */
+template <class F> static void host_parallel(int n_parts, F part)
+{
+    const int nt = std::min(8, n_parts);
+
+    if (nt <= 1) {
+        for (int k = 0; k < n_parts; k++)
+            part(k);
+        return;
+    }
+    std::vector<std::thread> th;
+
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            for (int k = t; k < n_parts; k += nt)
+                part(k);
+        });
+    for (auto &x : th)
+        x.join();
+}
+
+static void ensure_pinned(void **p, size_t *size, size_t bytes)
+{
+    if (*p && *size >= bytes)
+        return;
+    if (*p)
+        check(hipHostFree(*p), "hipHostFree");
+    *p = nullptr;
+    *size = 0;
+    check(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault),
+          "hipHostMalloc");
+    *size = bytes;
+}
+
+extern "C" void tcoh_run_region(struct ocl *o, int nrows, int ncols,
+                                const float *i, const float *q,
+                                const struct geometry *g, const double *heights,
+                                float *coh, int *count, float *pairs,
+                                int strip_rows)
+{
+    guarded_void([&] {
+        const struct settings &s = o->s;
+        const int n = s.ndates, ha = s.win_az / 2, hr = s.win_rg / 2;
+        const int np = s.npairs;
+        hipStream_t cs = o->stream;
+
+        if (nrows <= 0 || ncols <= 0)
+            fail("Invalid region of %d x %d pixels", nrows, ncols);
+        if (ncols + 2 * hr != o->padded_cols)
+            fail("Region of %d columns does not match the padded width %d",
+                 ncols, o->padded_cols);
+        if (!i || !q || !coh || !count || (np && !pairs))
+            fail("Region run without date planes or output buffers");
+        check(hipSetDevice(o->device), "hipSetDevice");
+        int R = strip_rows > 0 ? strip_rows : std::max(16, (1 << 20) / ncols);
+
+        R = std::min(R, nrows);
+        if (!o->region_ready) {
+            check(hipStreamCreateWithFlags(&o->copy, hipStreamNonBlocking),
+                  "hipStreamCreate");
+            for (int b = 0; b < 2; b++) {
+                hipEvent_t *evs[] = {&o->ev_h2d[b], &o->ev_prep[b], &o->ev_comp[b],
+                                     &o->ev_d2h[b]};
+                for (hipEvent_t *e : evs) {
+                    check(hipEventCreateWithFlags(e, hipEventDisableTiming),
+                          "hipEventCreate");
+                    check(hipEventRecord(*e, cs), "hipEventRecord");
+                }
+            }
+            o->region_ready = true;
+        }
+        const int hmax = R + 2 * ha;            /* staged plane rows */
+        const size_t plane_in = (size_t)hmax * ncols;
+        const size_t in_bytes = 2 * (size_t)n * plane_in * sizeof(float);
+        const size_t hgt_bytes = heights ? plane_in * sizeof(double) : 0;
+        const size_t out_px = (size_t)R * ncols;
+        const size_t out_bytes = (2 + (size_t)np) * out_px * sizeof(float);
+        const size_t pr_max = (size_t)R + s.win_az - 1;
+        const size_t npad = pr_max * o->padded_cols;
+        const double *dgrid = g ? upload_grid(o, g, nrows, ncols, cs) : nullptr;
+
+        for (int b = 0; b < 2; b++) {
+            /* buffers of an earlier call may still be in use */
+            check(hipEventSynchronize(o->ev_d2h[b]), "hipEventSynchronize");
+            check(hipEventSynchronize(o->ev_h2d[b]), "hipEventSynchronize");
+            o->rin[b].ensure(in_bytes, "hipMalloc(region planes)");
+            if (heights)
+                o->rhgt[b].ensure(hgt_bytes, "hipMalloc(region heights)");
+            o->rout[b].ensure(out_bytes, "hipMalloc(region outputs)");
+            ensure_pinned(&o->hin[b], &o->hin_size[b], in_bytes + hgt_bytes);
+            ensure_pinned(&o->hout[b], &o->hout_size[b], out_bytes);
+        }
+        check(hipStreamSynchronize(cs), "hipStreamSynchronize");
+        o->rslc.ensure(npad * n * 2 * sizeof(float), "hipMalloc(strip slc)");
+        o->ramp.ensure(npad * n * sizeof(float), "hipMalloc(strip amp)");
+        o->rvalid.ensure(npad, "hipMalloc(strip valid)");
+
+        const int nstrips = (nrows + R - 1) / R;
+        const size_t plane = (size_t)nrows * ncols;
+
+        /* outputs of strip t from pinned staging to the caller's arrays */
+        auto finish = [&](int t) {
+            const int b = t % 2, r0 = t * R, rs = std::min(R, nrows - r0);
+            const size_t px = (size_t)rs * ncols;
+            const float *h = (const float *)o->hout[b];
+
+            check(hipEventSynchronize(o->ev_d2h[b]), "hipEventSynchronize");
+            host_parallel(2 + np, [&](int k) {
+                const float *src = h + (size_t)k * px;
+
+                if (k == 0)
+                    memcpy(coh + (size_t)r0 * ncols, src, px * sizeof(float));
+                else if (k == 1)
+                    memcpy(count + (size_t)r0 * ncols, src, px * sizeof(int));
+                else
+                    memcpy(pairs + (size_t)(k - 2) * plane + (size_t)r0 * ncols,
+                           src, px * sizeof(float));
+            });
+        };
+
+        for (int t = 0; t < nstrips; t++) {
+            const int b = t % 2;
+            const int r0 = t * R, rs = std::min(R, nrows - r0);
+            const int a = std::max(0, r0 - ha), e = std::min(nrows, r0 + rs + ha);
+            const int hrows = e - a, prow = rs + s.win_az - 1;
+            const size_t pin = (size_t)hrows * ncols;
+            float *hbuf = (float *)o->hin[b];
+
+            /* 1. stage the strip's rows of every date (and heights) */
+            check(hipEventSynchronize(o->ev_h2d[b]), "hipEventSynchronize");
+            host_parallel(2 * n + (heights ? 1 : 0), [&](int k) {
+                if (k == 2 * n) {
+                    memcpy(hbuf + 2 * (size_t)n * pin, heights + (size_t)a * ncols,
+                           pin * sizeof(double));
+                    return;
+                }
+                const float *src = (k < n ? i : q) + (size_t)(k % n) * plane +
+                                   (size_t)a * ncols;
+
+                memcpy(hbuf + (size_t)k * pin, src, pin * sizeof(float));
+            });
+            /* 2. to the device once strip t - 2 has read the buffer */
+            check(hipStreamWaitEvent(o->copy, o->ev_prep[b], 0), "hipStreamWaitEvent");
+            check(hipMemcpyAsync(o->rin[b].p, hbuf, 2 * (size_t)n * pin * sizeof(float),
+                                 hipMemcpyHostToDevice, o->copy),
+                  "hipMemcpy(region planes)");
+            if (heights)
+                check(hipMemcpyAsync(o->rhgt[b].p, hbuf + 2 * (size_t)n * pin,
+                                     pin * sizeof(double), hipMemcpyHostToDevice,
+                                     o->copy),
+                      "hipMemcpy(region heights)");
+            check(hipEventRecord(o->ev_h2d[b], o->copy), "hipEventRecord");
+            /* 3. prepare and estimate on the compute stream, once the planes
+               are in and strip t - 2's outputs are out */
+            check(hipStreamWaitEvent(cs, o->ev_h2d[b], 0), "hipStreamWaitEvent");
+            check(hipStreamWaitEvent(cs, o->ev_d2h[b], 0), "hipStreamWaitEvent");
+            const float *di = o->rin[b].as<float>();
+            const StripArgs sa = strip_args(
+                o, hrows, ncols, di, di + (size_t)n * pin, g, dgrid,
+                heights ? o->rhgt[b].p : nullptr, (r0 - ha) - a, prow, hr, a,
+                o->rslc.p, o->ramp.p, o->rvalid.p);
+
+            check(launch_strip(sa, cs), "strip kernel");
+            check(hipEventRecord(o->ev_prep[b], cs), "hipEventRecord");
+            float *dout = o->rout[b].as<float>();
+            const size_t px = (size_t)rs * ncols;
+
+            run_device(o, o->rslc.p, o->ramp.p, o->rvalid.p, prow, rs, ncols, dout,
+                       dout + px, np ? dout + 2 * px : nullptr, cs);
+            check(hipEventRecord(o->ev_comp[b], cs), "hipEventRecord");
+            /* 4. outputs back on the copy stream */
+            check(hipStreamWaitEvent(o->copy, o->ev_comp[b], 0), "hipStreamWaitEvent");
+            check(hipMemcpyAsync(o->hout[b], dout, (2 + (size_t)np) * px * sizeof(float),
+                                 hipMemcpyDeviceToHost, o->copy),
+                  "hipMemcpy(region outputs)");
+            check(hipEventRecord(o->ev_d2h[b], o->copy), "hipEventRecord");
+            /* 5. the previous strip's outputs to the caller */
+            if (t > 0)
+                finish(t - 1);
+        }
+        finish(nstrips - 1);
+        check(hipStreamSynchronize(o->copy), "hipStreamSynchronize");
+        check(hipStreamSynchronize(cs), "hipStreamSynchronize");
     });
 }
 
@@ -636,9 +854,28 @@ extern "C" void ocl_close(struct ocl *o)
         DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
                           &o->count, &o->pairs, &o->ampT,  &o->lstat,
                           &o->mask, &o->adtab, &o->hl,
-                          &o->absmax, &o->grid};
+                          &o->absmax, &o->grid, &o->rin[0], &o->rin[1],
+                          &o->rhgt[0], &o->rhgt[1], &o->rout[0], &o->rout[1],
+                          &o->rslc, &o->ramp, &o->rvalid};
+        if (o->copy)
+            (void)hipStreamSynchronize(o->copy);
         for (DevBuf *b : bufs)
             b->release();
+        for (int b = 0; b < 2; b++) {
+            if (o->hin[b])
+                (void)hipHostFree(o->hin[b]);
+            if (o->hout[b])
+                (void)hipHostFree(o->hout[b]);
+        }
+        if (o->region_ready) {
+            for (int b = 0; b < 2; b++) {
+                (void)hipEventDestroy(o->ev_h2d[b]);
+                (void)hipEventDestroy(o->ev_prep[b]);
+                (void)hipEventDestroy(o->ev_comp[b]);
+                (void)hipEventDestroy(o->ev_d2h[b]);
+            }
+            (void)hipStreamDestroy(o->copy);
+        }
         o->timer.release();
         (void)hipStreamDestroy(o->stream);
         delete o;

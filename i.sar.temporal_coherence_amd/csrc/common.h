@@ -355,7 +355,10 @@ struct StripArgs {
     float *amp;               /* [padded pixel][n], increasing */
     unsigned char *valid;     /* [padded pixel] */
     int n, nrows, ncols;      /* dates, region size */
-    int row0, prow, halo, pc; /* first region row, padded rows, halo, width */
+    int row0, prow, halo, pc; /* first padded row (plane rows), padded rows,
+                                 halo, padded width */
+    int plane_row0;           /* region row of plane row 0 (the planes may
+                                 hold a band of the region's rows) */
     int grid_cols, step, nh;  /* phase grid */
     double hx[3], h0;         /* grid heights, height without elevation */
 };

@@ -176,6 +176,23 @@ void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
                                int row0, int padded_rows, int halo_cols,
                                void *slc, void *amp, void *valid, void *stream);
 
+/* The whole region from host date planes, streamed strip by strip: the
+   strip loop of main.c:724-825 with ocl_run (ocl.c:289-374) pipelined. Each
+   strip's rows (plus the window halo) of every date are staged in pinned
+   memory and copied on a second stream while the previous strip computes;
+   the strip is prepared on the device (tcoh_prepare_strip_device, so g and
+   heights as there, heights on the host here), estimated, and its maps
+   copied back while the next strip computes. i, q: float[n][nrows][ncols];
+   outputs row-major nrows x ncols, pairs planar [npairs][nrows][ncols]
+   (may be NULL without pairs). ncols + win_rg - 1 must be the handle's
+   padded width. strip_rows: output rows per strip, 0 for about a million
+   pixels. The maps equal a single-strip run's (strips are seamless).
+   Synchronous. Extension: the reference streams nothing. */
+void tcoh_run_region(struct ocl *o, int nrows, int ncols, const float *i,
+                     const float *q, const struct geometry *g,
+                     const double *heights, float *coh, int *count,
+                     float *pairs, int strip_rows);
+
 /* ocl_run on device-resident buffers (hipMalloc'd, same layouts), enqueued
    on stream (a hipStream_t; NULL is the handle's own stream). Returns
    without synchronising. */

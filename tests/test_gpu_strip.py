@@ -145,3 +145,64 @@ def test_strip_errors_are_reported(tcoh):
         g = tcoh.api.make_geometry(grid, [0.0])
         with pytest.raises(RuntimeError, match="does not cover"):
             be.prepare_strip_device(p, p, rows, cols, 0, 1, 2, p, p, p, geometry=g)
+
+
+def same_maps(got, want):
+    for g, w in zip(got, want):
+        if w is None:
+            assert g is None
+            continue
+        assert np.array_equal(np.asarray(g).view(np.uint32), np.asarray(w).view(np.uint32))
+
+
+@pytest.mark.parametrize("precision", ["plain", "f16"])
+def test_region_stream_equals_one_strip(tcoh, cref, precision):
+    """tcoh_run_region (strips staged, prepared and estimated in a
+    pipeline) gives the maps of one ocl_run over the whole host-prepared
+    region, bit for bit, for several strip heights (reference
+    test:118-135: strips are seamless)."""
+    n, rows, cols = 8, 70, 33
+    ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
+    i, q = planes(tcoh, n, rows, cols, seed=31)
+    s = tcoh.make_settings(n, window=WINDOW, min_shp=9, pairs="all", bias=True)
+    prow, pc = rows + 2 * ha, cols + 2 * hr
+    slc_h, amp_h, valid_h = cref.prepare_strip(i, q, -ha, prow, hr)
+    with tcoh.Backend(s, pc, platform=0, device=0, precision=precision) as be:
+        want = be.run(slc_h, amp_h, valid_h, prow, rows, cols)
+        for strip_rows in (16, 23, 0):
+            same_maps(be.run_region(i, q, strip_rows=strip_rows), want)
+    assert np.sum(want[1] >= s.min_looks) > 100
+
+
+def test_region_stream_with_geometric_phase(tcoh, cref):
+    """With the flat-earth / topographic phase removed at elevation heights:
+    the streamed region equals the whole region prepared on the device
+    (the same strip kernel over the region's full planes) and estimated in
+    one strip."""
+    n, rows, cols = 6, 58, 40
+    ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
+    i, q = planes(tcoh, n, rows, cols, seed=41)
+    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, seed=6)
+    grid, hs, status = cref.geometry_grid(orbits, scene, ref, 0.0, 400.0)
+    assert status == 0
+    heights = np.random.default_rng(7).uniform(0.0, 400.0, (rows, cols))
+    heights[10, 11] = np.nan
+    g = tcoh.api.make_geometry(grid, hs, h0=50.0)
+    s = tcoh.make_settings(n, window=WINDOW, min_shp=9, pairs="consecutive")
+    prow, pc = rows + 2 * ha, cols + 2 * hr
+    D = tcoh.api.DeviceBuffer
+    with tcoh.Backend(s, pc, platform=0, device=0) as be:
+        di, dq, dh = D.from_array(i), D.from_array(q), D.from_array(heights)
+        slc, amp, valid = D(prow * pc * n * 8), D(prow * pc * n * 4), D(prow * pc)
+        coh, count = D(rows * cols * 4), D(rows * cols * 4)
+        pairs = D(s.npairs * rows * cols * 4)
+        be.prepare_strip_device(di.ptr, dq.ptr, rows, cols, -ha, prow, hr, slc.ptr,
+                                amp.ptr, valid.ptr, geometry=g, heights_ptr=dh.ptr)
+        be.run_device(slc.ptr, amp.ptr, valid.ptr, prow, rows, cols, coh.ptr,
+                      count.ptr, pairs.ptr)
+        want = (coh.to_array(np.float32), count.to_array(np.int32),
+                pairs.to_array(np.float32, (s.npairs, rows * cols)))
+        for strip_rows in (13, 0):
+            same_maps(be.run_region(i, q, geometry=g, heights=heights,
+                                    strip_rows=strip_rows), want)
+    assert want[1][10 * cols + 11] == -1
