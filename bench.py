@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpixel/s at N=30, 21x7 KS+EVD: mixed-prec vs plain FP32 GPU, |dγ|<=1e-4"
 METRIC_EMI = "Mpixel/s at N=30, 21x7 AD+EMI, all pairs: mixed-prec vs plain FP32 GPU"
+METRIC_GEO = ("Mpixel/s at N=60, 21x7 KS+EVD, geometric phase removed: mixed-prec vs "
+              "plain FP32 GPU, one GPU's 1024 x 25000 share")
 PEAK_FP64 = 78.6e12    # MI355X FP64 vector (TFLOP/s)
 PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
 PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
@@ -41,18 +43,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def prepare_on_device(tcoh, torch, s, stack, ha, hr):
+def prepare_on_device(tcoh, torch, s, i, q, ha, hr, geometry=None, heights=None):
     """read_strip (main.c:52-127) on the GPU for one whole-frame strip, by
     the library's strip kernel (tcoh_prepare_strip_device) from the date
-    planes; returns the strip buffers and the kernel's device time (ms,
-    HIP events on the stream it runs on, second of two calls)."""
-    n, rows, cols = stack.shape
+    planes i, q (n, rows, cols) float32 on the device, with the geometric
+    phase of `geometry` at `heights` (rows, cols) float64 removed when
+    given; returns the strip buffers and the kernel's device time (ms, HIP
+    events on the stream it runs on, second of two calls)."""
+    n, rows, cols = i.shape
     pr, pc = rows + 2 * ha, cols + 2 * hr
-    i = stack.real.contiguous()
-    q = stack.imag.contiguous()
-    slc = torch.empty((pr, pc, n, 2), dtype=torch.float32, device=stack.device)
-    amp = torch.empty((pr, pc, n), dtype=torch.float32, device=stack.device)
-    valid = torch.empty((pr, pc), dtype=torch.uint8, device=stack.device)
+    slc = torch.empty((pr, pc, n, 2), dtype=torch.float32, device=i.device)
+    amp = torch.empty((pr, pc, n), dtype=torch.float32, device=i.device)
+    valid = torch.empty((pr, pc), dtype=torch.uint8, device=i.device)
     # a stream of its own: a null handle would mean the library handle's
     # stream, which events recorded by torch would not bracket
     st = torch.cuda.Stream()
@@ -64,10 +66,58 @@ def prepare_on_device(tcoh, torch, s, stack, ha, hr):
             e0.record(st)
             be.prepare_strip_device(i.data_ptr(), q.data_ptr(), rows, cols, -ha, pr, hr,
                                     slc.data_ptr(), amp.data_ptr(), valid.data_ptr(),
+                                    geometry=geometry,
+                                    heights_ptr=None if heights is None else
+                                    heights.data_ptr(),
                                     stream=st.cuda_stream)
             e1.record(st)
         torch.cuda.synchronize()
     return slc, amp, valid, e0.elapsed_time(e1)
+
+
+def config5_inputs(tcoh, torch, s, n, rows, cols, hr):
+    """The stack of the geometry workload: the Voronoi decorrelation scene
+    times exp(-i phi_geo), phi_geo the flat-earth and topographic phase of
+    60 dates (circular polar orbits 700 km up, baselines ~ N(0, 100 m),
+    reference date 0, C band) at a smooth random DEM of 0-500 m (about 2 km
+    correlation at 20 m pixels). The phase grid comes from the library's
+    FP64 geometry, and the library's own strip kernel applies the forward
+    rotation (the grid negated). Returns I/Q planes on the GPU, the heights
+    on the GPU and the geometry struct the timed preparation removes."""
+    import torch.nn.functional as F
+    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
+                                            gamma0=(0.05, 0.95))
+    stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
+    i = stack.real.contiguous()
+    q = stack.imag.contiguous()
+    del stack
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    coarse = torch.rand((1, 1, rows // 100 + 3, cols // 100 + 3), generator=gen,
+                        device="cuda", dtype=torch.float64)
+    dem = F.interpolate(coarse, size=(rows, cols), mode="bicubic", align_corners=True)[0, 0]
+    dem = (dem - dem.min()) / (dem.max() - dem.min()) * 500.0
+    heights = dem.contiguous()
+    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, ref=0, seed=5)
+    grid, hs = tcoh.api.geometry_grid(orbits, scene, ref, 0.0, 500.0)
+    g = tcoh.api.make_geometry(grid, hs)
+    fwd = tcoh.api.make_geometry(-grid, hs)
+    pc = cols + 2 * hr
+    slc = torch.empty((rows, pc, n, 2), dtype=torch.float32, device="cuda")
+    amp = torch.empty((rows, pc, n), dtype=torch.float32, device="cuda")
+    valid = torch.empty((rows, pc), dtype=torch.uint8, device="cuda")
+    with tcoh.Backend(s, pc, platform=0, device=0) as be:
+        be.prepare_strip_device(i.data_ptr(), q.data_ptr(), rows, cols, 0, rows, hr,
+                                slc.data_ptr(), amp.data_ptr(), valid.data_ptr(),
+                                geometry=fwd, heights_ptr=heights.data_ptr())
+        torch.cuda.synchronize()
+    del amp, valid
+    obs = slc[:, hr:hr + cols].permute(2, 3, 0, 1)
+    i = obs[:, 0].contiguous()
+    q = obs[:, 1].contiguous()
+    del slc, obs
+    torch.cuda.synchronize()
+    return i, q, heights, g
 
 
 def flops_eigen(n):
@@ -123,11 +173,11 @@ def pixel_conditioning(fp64, slc_c, count, mask, sd, pcols, size, k):
     return out
 
 
-def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, s_opts, y0, x0, size,
+def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, n, s_opts, y0, x0, size,
                   gpu_maps):
     """Oracle SHP counts and FP64 gamma/pairs on a size x size crop; the
     FP32 C oracle's own gamma error beside the GPU forms'."""
-    n, wa, wr = 30, 21, 7
+    wa, wr = s_opts["window"]
     pr_c, pc_c = size + wa - 1, size + wr - 1
     slc3 = slc.reshape(-1, pcols, n * 2)[y0:y0 + pr_c, x0:x0 + pc_c]
     amp3 = amp.reshape(-1, pcols, n)[y0:y0 + pr_c, x0:x0 + pc_c]
@@ -145,7 +195,7 @@ def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, s_opts, y0, x0, size
     for form, (coh, count, pairs) in gpu_maps.items():
         cc = coh[y0:y0 + size, x0:x0 + size].reshape(-1)
         kc = count[y0:y0 + size, x0:x0 + size].reshape(-1)
-        pc = pairs[:, y0:y0 + size, x0:x0 + size].reshape(pairs.shape[0], -1).T
+        pc = pairs[:, y0:y0 + size, x0:x0 + size].reshape(pairs.shape[0], size * size).T
         d = np.abs(cc - c64)
         dp = np.abs(pc - p64)
         worst = int(np.nanargmax(d))
@@ -155,7 +205,7 @@ def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, s_opts, y0, x0, size
             "rms_dgamma": float(np.sqrt(np.nanmean(d ** 2))),
             "pixels_over_1e-4": int(np.sum(d > 1e-4)),
             "max_abs_dgamma_off_bias_threshold": float(np.nanmax(off)),
-            "max_abs_dpair": float(np.nanmax(dp)),
+            "max_abs_dpair": float(np.nanmax(dp)) if dp.size else None,
             "nan_mask_equal": bool(np.array_equal(np.isnan(cc), np.isnan(c64))),
             "shp_count_mismatches": int(np.sum(kc != ocount)),
             "worst": dict(pixel_conditioning(fp64, slc_c, ocount, mask, sd, pc_c,
@@ -172,18 +222,24 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="f16", choices=["f16", "bf16"])
-    ap.add_argument("--rows", type=int, default=2048)
-    ap.add_argument("--cols", type=int, default=2048)
+    ap.add_argument("--rows", type=int, default=None,
+                    help="default 2048 (configs 2, 3), 1024 (config 5)")
+    ap.add_argument("--cols", type=int, default=None,
+                    help="default 2048 (configs 2, 3), 25000 (config 5)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=384)
+    ap.add_argument("--cpu-rows", type=int, default=None,
+                    help="rows of the CPU baseline band (default 384, 64, 4 for "
+                         "configs 2, 3, 5)")
     ap.add_argument("--crop", type=int, default=256)
     ap.add_argument("--no-region", action="store_true",
                     help="skip the host-to-host rate through tcoh_run_region")
-    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
-                    help="2: KS+EVD, bias, consecutive pairs, gamma0 ~ U(0.05, 0.95) "
-                         "(the headline); 3: AD+EMI, bias, all 435 pairs, "
-                         "gamma0 ~ U(0.6, 0.99)")
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 5],
+                    help="2: N=30 KS+EVD, bias, consecutive pairs, gamma0 ~ U(0.05, "
+                         "0.95) (the headline); 3: N=30 AD+EMI, bias, all 435 pairs, "
+                         "gamma0 ~ U(0.6, 0.99); 5: N=60 KS+EVD, bias, flat-earth and "
+                         "topographic phase removed at DEM heights, one GPU's share "
+                         "(1024 x 25000) of 8192 x 25000")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device three times: a kernel of "
                          "known traffic to calibrate HBM counters against")
@@ -199,10 +255,20 @@ def main():
     from oracle import cref, fp64
 
     tcoh = ge.load_package()
-    n, rows, cols = 30, args.rows, args.cols
+    c5 = args.config == 5
+    n = 60 if c5 else 30
+    rows = args.rows or (1024 if c5 else 2048)
+    cols = args.cols or (25000 if c5 else 2048)
+    cpu_rows = args.cpu_rows or {2: 384, 3: 64, 5: 4}[args.config]
     wa, wr = 21, 7
     ha, hr = wa // 2, wr // 2
-    if args.config == 2:
+    if c5:
+        s_opts = dict(window=(wa, wr), shp_test="ks", alpha=0.05, min_shp=20,
+                      estimator="evd", bias=True, pairs=None)
+        gamma0, label = (0.05, 0.95), "KS alpha 0.05, EVD, min_shp 20 (min_looks 60), " \
+                                      "bias correction, no pair maps; flat-earth and " \
+                                      "topographic phase of a 0-500 m DEM removed"
+    elif args.config == 2:
         s_opts = dict(window=(wa, wr), shp_test="ks", alpha=0.05, min_shp=30,
                       estimator="evd", bias=True, pairs="consecutive")
         gamma0, label = (0.05, 0.95), "KS alpha 0.05, EVD, min_shp 30, bias correction, " \
@@ -225,15 +291,22 @@ def main():
         del src, dst
 
     t0 = time.perf_counter()
-    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
-                                            gamma0=gamma0)
-    stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
-    slc, amp, valid, prep_ms = prepare_on_device(tcoh, torch, s, stack, ha, hr)
+    geo, dheights = None, None
+    if c5:
+        di, dq, dheights, geo = config5_inputs(tcoh, torch, s, n, rows, cols, hr)
+    else:
+        reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
+                                                gamma0=gamma0)
+        stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
+        di, dq = stack.real.contiguous(), stack.imag.contiguous()
+        del stack
+    slc, amp, valid, prep_ms = prepare_on_device(tcoh, torch, s, di, dq, ha, hr,
+                                                 geometry=geo, heights=dheights)
     host_planes = None
     if not args.no_region:
-        host_planes = (stack.real.contiguous().cpu().numpy(),
-                       stack.imag.contiguous().cpu().numpy())
-    del stack
+        host_planes = (di.cpu().numpy(), dq.cpu().numpy(),
+                       None if dheights is None else dheights.cpu().numpy())
+    del di, dq
     pcols = cols + 2 * hr
     coh = torch.empty((rows, cols), dtype=torch.float32, device="cuda")
     count = torch.empty((rows, cols), dtype=torch.int32, device="cuda")
@@ -286,7 +359,7 @@ def main():
                 "launches_per_step": float(nl[dom])}
 
     out = {
-        "metric": METRIC if args.config == 2 else METRIC_EMI,
+        "metric": {2: METRIC, 3: METRIC_EMI, 5: METRIC_GEO}[args.config],
         "value": value,
         "unit": "Mpixel/s",
         "n_gpus": 1,
@@ -299,9 +372,9 @@ def main():
         "dtype": "f16 hi/lo split, f32 accumulate" if args.precision == "f16"
                  else "bf16 hi/lo split, f32 accumulate",
         "data": "synthetic",
-        "config": {"workload": "N=30 dates, %dx%d px, 21x7 window, %s; Voronoi "
+        "config": {"workload": "N=%d dates, %dx%d px, 21x7 window, %s; Voronoi "
                                "decorrelation scene (16 cells, gamma0 ~ U(%g, %g)) seed 0"
-                               % (rows, cols, label, gamma0[0], gamma0[1]),
+                               % (n, rows, cols, label, gamma0[0], gamma0[1]),
                    "estimated_pixels": est, "mean_shp": float(looks.mean()),
                    "strip_prep_ms": round(prep_ms, 3),
                    "stage_ms": dict(zip(stage_names, np.round(st_ms, 3).tolist()))},
@@ -319,14 +392,15 @@ def main():
         # point (pinned staging, copies overlapping computation, strips of
         # about a million pixels): the rate a caller holding the stack in
         # host memory sees, PCIe transfers included. Not the headline value.
-        hi, hq = host_planes
+        hi, hq, hh = host_planes
         with tcoh.Backend(s, pcols, platform=0, device=0,
                           precision=args.precision) as be:
-            be.run_region(hi, hq)
+            reps = 1 if c5 else 2
+            if not c5:
+                be.run_region(hi, hq, geometry=geo, heights=hh)
             t0 = time.perf_counter()
-            reps = 2
             for _ in range(reps):
-                rc, rn, rp = be.run_region(hi, hq)
+                rc, rn, rp = be.run_region(hi, hq, geometry=geo, heights=hh)
             dt = (time.perf_counter() - t0) / reps
         same = bool(np.array_equal(rn.reshape(rows, cols), maps[args.precision][1]) and
                     np.array_equal(rc.reshape(rows, cols).view(np.uint32),
@@ -334,10 +408,11 @@ def main():
         out["host_to_host"] = {
             "value": npix / dt / 1e6, "unit": "Mpixel/s", "ms_per_frame": dt * 1e3,
             "path": "tcoh_run_region: I/Q planes in host memory -> maps in host memory",
-            "bytes_in": int(2 * hi.nbytes), "maps_equal_device_run": same}
+            "bytes_in": int(2 * hi.nbytes + (0 if hh is None else hh.nbytes)),
+            "maps_equal_device_run": same}
         log("host to host: %.1f ms per frame (%.1f Mpx/s), maps equal: %s" %
             (dt * 1e3, npix / dt / 1e6, same))
-        del host_planes, hi, hq
+        del host_planes, hi, hq, hh
 
     if args.crop > 0:
         t0 = time.perf_counter()
@@ -346,12 +421,12 @@ def main():
         size = min(args.crop, rows, cols)
         out["accuracy"] = crop_accuracy(tcoh, cref, fp64, slc.cpu().numpy(),
                                         amp.cpu().numpy(), valid.cpu().numpy(), pcols,
-                                        s_opts, y0, x0, size, maps)
+                                        n, s_opts, y0, x0, size, maps)
         out["accuracy"]["crop"] = [y0, x0, size]
         log("accuracy crop in %.1f s" % (time.perf_counter() - t0))
 
     if not args.no_cpu:
-        band = min(args.cpu_rows, rows)
+        band = min(cpu_rows, rows)
         threads = min(16, os.cpu_count() or 1)
         so = cref.settings(n, **s_opts)
         pr_b = band + wa - 1

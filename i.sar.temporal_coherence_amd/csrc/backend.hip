@@ -655,13 +655,13 @@ extern "C" void tcoh_prepare_strip_device(struct ocl *o, int nrows, int ncols,
     });
 }
 
-/* Host copies of n_parts equal-sized parts (dst_k <- src_k, bytes each)
-   on up to 8 threads: the pinned staging copies of tcoh_run_region run
-   while the GPU computes, and one thread alone would not keep up with
-   the device. */
+/* Parts 0 .. n_parts-1 of a host copy on up to 16 threads (at most the
+   hardware's): the pinned staging copies of tcoh_run_region run while the
+   GPU computes, and one thread alone would not keep up with the device. */
 template <class F> static void host_parallel(int n_parts, F part)
 {
-    const int nt = std::min(8, n_parts);
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::min({16, hw, n_parts});
 
     if (nt <= 1) {
         for (int k = 0; k < n_parts; k++)
