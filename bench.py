@@ -34,6 +34,10 @@ METRIC = "Mpixel/s at N=30, 21x7 KS+EVD: mixed-prec vs plain FP32 GPU, |dγ|<=1e
 METRIC_EMI = "Mpixel/s at N=30, 21x7 AD+EMI, all pairs: mixed-prec vs plain FP32 GPU"
 METRIC_GEO = ("Mpixel/s at N=60, 21x7 KS+EVD, geometric phase removed: mixed-prec vs "
               "plain FP32 GPU, one GPU's 1024 x 25000 share")
+# HBM bytes per k_link launch (512 rows x 2048 px, split FP16, N=30) from
+# the two counter passes of profiles/k_link_n30_2048x2048_kernel_stats.md:
+# (287,726 KiB x 2.0 + 715,473 KiB x 1.0) x 1024 B.
+TRAFFIC_F16_N30 = (287726 * 2.0 + 715473 * 1.0) * 1024
 PEAK_FP64 = 78.6e12    # MI355X FP64 vector (TFLOP/s)
 PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
 PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
@@ -354,7 +358,12 @@ def main():
     achieved = flop / max(1.0, nl[dom]) / avg_launch_s
     peak = flop / t_bound
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": TRAFFIC_F16_N30 if (args.config == 2 and args.precision == "f16"
+                                               and rows == 2048 and cols == 2048) else None,
+                "traffic_source": "profiles/k_link_n30_2048x2048_kernel_stats.md "
+                                  "(rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
+                                  "calibrated on a 1 GiB copy)",
                 "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,
                 "launches_per_step": float(nl[dom])}
 
