@@ -119,8 +119,24 @@ template <int CTRL> __device__ __forceinline__ int dpp_i(int v)
     return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
 
+/* A whole-wavefront group (G = 64): each 16-lane row is reduced with the
+   four DPP steps above, then the four row results are read into scalar
+   registers (v_readlane) and combined there; every lane gets the total,
+   and no LDS round trip (ds_bpermute) is on the path. */
+__device__ __forceinline__ float rows4(float v, int r)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), r));
+}
+
 template <int G> __device__ __forceinline__ float gsum(float v)
 {
+    if (G == 64) {
+        v += dpp_f<DPP_XOR1>(v);
+        v += dpp_f<DPP_XOR2>(v);
+        v += dpp_f<DPP_HALF_MIRROR>(v);
+        v += dpp_f<DPP_MIRROR>(v);
+        return (rows4(v, 0) + rows4(v, 16)) + (rows4(v, 32) + rows4(v, 48));
+    }
     if (G <= 16) {
         if (G >= 2)
             v += dpp_f<DPP_XOR1>(v);
@@ -148,6 +164,13 @@ template <int G> __device__ __forceinline__ double gsumd(double v)
 
 template <int G> __device__ __forceinline__ float gmax(float v)
 {
+    if (G == 64) {
+        v = fmaxf(v, dpp_f<DPP_XOR1>(v));
+        v = fmaxf(v, dpp_f<DPP_XOR2>(v));
+        v = fmaxf(v, dpp_f<DPP_HALF_MIRROR>(v));
+        v = fmaxf(v, dpp_f<DPP_MIRROR>(v));
+        return fmaxf(fmaxf(rows4(v, 0), rows4(v, 16)), fmaxf(rows4(v, 32), rows4(v, 48)));
+    }
     if (G <= 16) {
         if (G >= 2)
             v = fmaxf(v, dpp_f<DPP_XOR1>(v));
@@ -167,6 +190,13 @@ template <int G> __device__ __forceinline__ float gmax(float v)
 
 template <int G> __device__ __forceinline__ float gmin(float v)
 {
+    if (G == 64) {
+        v = fminf(v, dpp_f<DPP_XOR1>(v));
+        v = fminf(v, dpp_f<DPP_XOR2>(v));
+        v = fminf(v, dpp_f<DPP_HALF_MIRROR>(v));
+        v = fminf(v, dpp_f<DPP_MIRROR>(v));
+        return fminf(fminf(rows4(v, 0), rows4(v, 16)), fminf(rows4(v, 32), rows4(v, 48)));
+    }
     if (G <= 16) {
         if (G >= 2)
             v = fminf(v, dpp_f<DPP_XOR1>(v));
@@ -186,6 +216,14 @@ template <int G> __device__ __forceinline__ float gmin(float v)
 
 template <int G> __device__ __forceinline__ int gmaxi(int v)
 {
+    if (G == 64) {
+        v = max(v, dpp_i<DPP_XOR1>(v));
+        v = max(v, dpp_i<DPP_XOR2>(v));
+        v = max(v, dpp_i<DPP_HALF_MIRROR>(v));
+        v = max(v, dpp_i<DPP_MIRROR>(v));
+        return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+                   max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+    }
     if (G <= 16) {
         if (G >= 2)
             v = max(v, dpp_i<DPP_XOR1>(v));
@@ -205,6 +243,14 @@ template <int G> __device__ __forceinline__ int gmaxi(int v)
 
 template <int G> __device__ __forceinline__ int gori(int v)
 {
+    if (G == 64) {
+        v |= dpp_i<DPP_XOR1>(v);
+        v |= dpp_i<DPP_XOR2>(v);
+        v |= dpp_i<DPP_HALF_MIRROR>(v);
+        v |= dpp_i<DPP_MIRROR>(v);
+        return (__builtin_amdgcn_readlane(v, 0) | __builtin_amdgcn_readlane(v, 16)) |
+               (__builtin_amdgcn_readlane(v, 32) | __builtin_amdgcn_readlane(v, 48));
+    }
     if (G <= 16) {
         if (G >= 2)
             v |= dpp_i<DPP_XOR1>(v);

@@ -51,10 +51,17 @@
 
 /* Per-pixel LDS of the eigen solve: the tridiagonal (d, e) and tau of the
    reduction, the inverse iteration's LU factors (u0 holds 1/pivot) and
-   result y, the phasors z. */
+   result y, the phasors z. In groups of 64 lanes (depths above 32) the
+   reduction broadcasts its row operands through vw[r] = (v_r, w_r) (one
+   16-byte word a row), which the inverse iteration's arrays replace. */
 template <int N> struct EigLds {
-    float u0[N], u1[N], u2[N], l[N], y[N];
-    int piv[N];
+    union {
+        float4 vw[N];
+        struct {
+            float u0[N], u1[N], u2[N], l[N], y[N];
+            int piv[N];
+        };
+    };
     float2 tau[N];
     float d[N], e[N], e2[N]; /* e2 = e * e, rounded as the Sturm count's */
     float2 z[N];
@@ -139,6 +146,40 @@ template <int L> struct RowOps {
                      : "v"(vx), "v"(vy), "v"(ax), "v"(ay), "n"(L));
     }
 };
+
+/* One row r > k of a 64-lane group's reduction, with (v_r, w_r) read from
+   LDS as t: the matrix-vector partial sums, and the rank-2 update (signs on
+   the lane's own operands, as in the broadcast form). */
+template <int N>
+__device__ __forceinline__ void matvec_row64(int j, int k, float4 t, const float2 (&a)[N],
+                                             float &sr, float &sia, float &sib)
+{
+    if (j <= k || j >= N)
+        return;
+    sr = fmaf(t.x, a[j].x, sr);
+    sr = fmaf(t.y, a[j].y, sr);
+    sia = fmaf(t.y, a[j].x, sia);
+    sib = fmaf(t.x, a[j].y, sib);
+}
+
+template <int N>
+__device__ __forceinline__ void update_row64(int r, int k, float4 t, float2 (&a)[N],
+                                             float2 nw, float2 nv, float wcy, float vcy)
+{
+    if (r <= k || r >= N)
+        return;
+    float ax = a[r].x, ay = a[r].y;
+
+    ax = fmaf(t.x, nw.x, ax);
+    ax = fmaf(t.y, nw.y, ax);
+    ax = fmaf(t.z, nv.x, ax);
+    ax = fmaf(t.w, nv.y, ax);
+    ay = fmaf(t.y, nw.x, ay);
+    ay = fmaf(t.x, wcy, ay);
+    ay = fmaf(t.w, nv.x, ay);
+    ay = fmaf(t.z, vcy, ay);
+    a[r] = make_float2(ax, ay);
+}
 
 #define TCOH_ROW_CASES(F)                                                      \
     F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13)  \
@@ -264,9 +305,47 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             sia[m] = 0.0f;
             sib[m] = 0.0f;
         }
+        if constexpr (G == 64) {
+            /* lane c publishes v_c; rows are read back as LDS broadcasts
+               (a wavefront-wide readlane per value would cost a VALU
+               instruction each and tie up scalar registers) */
+            const int c = lg;
+
+            if (c >= k + 1 && c < N) {
+                sh.vw[c].x = vc[0].x;
+                sh.vw[c].y = vc[0].y;
+            }
+            wave_sync();
+            /* four rows in flight: rows j + 4 .. j + 7 load while rows
+               j .. j + 3 compute (named values carried across the unrolled
+               loop, so that no array is indexed before unrolling) */
+            float4 c0, c1, c2, c3;
+
+            c0 = (0 > k && 0 < N) ? sh.vw[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c1 = (1 > k && 1 < N) ? sh.vw[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c2 = (2 > k && 2 < N) ? sh.vw[2] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c3 = (3 > k && 3 < N) ? sh.vw[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j < N; j += 4) {
+                const float4 n0 = (j + 4 > k && j + 4 < N) ? sh.vw[j + 4] : c0;
+                const float4 n1 = (j + 5 > k && j + 5 < N) ? sh.vw[j + 5] : c1;
+                const float4 n2 = (j + 6 > k && j + 6 < N) ? sh.vw[j + 6] : c2;
+                const float4 n3 = (j + 7 > k && j + 7 < N) ? sh.vw[j + 7] : c3;
+
+                matvec_row64<N>(j, k, c0, A[0], sr2[0], sia[0], sib[0]);
+                matvec_row64<N>(j + 1, k, c1, A[0], sr2[0], sia[0], sib[0]);
+                matvec_row64<N>(j + 2, k, c2, A[0], sr2[0], sia[0], sib[0]);
+                matvec_row64<N>(j + 3, k, c3, A[0], sr2[0], sia[0], sib[0]);
+                TCOH_STEP_FENCE();
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < N; j++) {
-            if (j <= k)
+            if (G == 64 || j <= k)
                 continue;
             if (G == 16) {
 #pragma unroll
@@ -328,9 +407,43 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             nw[m] = make_float2(-wc[m].x, -wc[m].y);
             nv[m] = make_float2(-vc[m].x, -vc[m].y);
         }
+        if constexpr (G == 64) {
+            const int c = lg;
+
+            if (c >= k + 1 && c < N) {
+                sh.vw[c].z = wc[0].x;
+                sh.vw[c].w = wc[0].y;
+            }
+            wave_sync();
+            float4 c0, c1, c2, c3;
+
+            c0 = (0 > k && 0 < N) ? sh.vw[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c1 = (1 > k && 1 < N) ? sh.vw[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c2 = (2 > k && 2 < N) ? sh.vw[2] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c3 = (3 > k && 3 < N) ? sh.vw[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int r = 0; r < N; r += 4) {
+                const float4 n0 = (r + 4 > k && r + 4 < N) ? sh.vw[r + 4] : c0;
+                const float4 n1 = (r + 5 > k && r + 5 < N) ? sh.vw[r + 5] : c1;
+                const float4 n2 = (r + 6 > k && r + 6 < N) ? sh.vw[r + 6] : c2;
+                const float4 n3 = (r + 7 > k && r + 7 < N) ? sh.vw[r + 7] : c3;
+
+                update_row64<N>(r, k, c0, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
+                update_row64<N>(r + 1, k, c1, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
+                update_row64<N>(r + 2, k, c2, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
+                update_row64<N>(r + 3, k, c3, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
+                TCOH_STEP_FENCE();
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+            /* vw is read again only after the next step's writes */
+            wave_sync();
+        }
 #pragma unroll
         for (int r = 0; r < N; r++) {
-            if (r <= k)
+            if (G == 64 || r <= k)
                 continue;
             if (G == 16) {
 #pragma unroll
