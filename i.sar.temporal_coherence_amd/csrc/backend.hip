@@ -172,6 +172,8 @@ struct ocl {
     DevBuf slc, amp, valid, coh, count, pairs;
     /* scratch */
     DevBuf ampT, lstat, mask, adtab, hl, absmax;
+    /* EMI pixels deferred to the Jacobi queue kernel: T, pixel, count */
+    DevBuf qt, qpix, qn;
     int adtab_n;
     /* the phase grid of tcoh_prepare_strip_device / tcoh_run_region */
     DevBuf grid;
@@ -203,8 +205,11 @@ static bool eig_supported(int n);
 static hipError_t launch_link(const KParams &kp, const float2 *slc,
                               const uint2 *hl, const uint32_t *mask,
                               const int *count, float *coh, float *pairs,
-                              int row0, int nrows, int precision,
-                              hipStream_t s);
+                              int row0, int nrows, int precision, float2 *qt,
+                              int *qpix, int *qn, hipStream_t s);
+static hipError_t launch_emi_queue(const KParams &kp, const float2 *qt,
+                                   const int *qpix, int qcount, float *coh,
+                                   hipStream_t s);
 
 extern "C" void ocl_list_devices(void)
 {
@@ -472,19 +477,49 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
     }
 
     /* Fused covariance + linking, in launches of about a million output
-       pixels (each well under the GPU watchdog, ocl.c:22-23). */
+       pixels (each well under the GPU watchdog, ocl.c:22-23). EMI at
+       depths up to 32 defers the pixels that need the Jacobi
+       pseudo-inverse to a queue and links them in a second launch, four
+       to a wavefront. */
     const int chunk = std::max(1, (1 << 20) / cols);
+    const bool queue = kp.emi && n <= 32;
+    const size_t nt = (size_t)n * (n - 1) / 2;
 
+    if (queue) {
+        const size_t cap = (size_t)std::min(chunk, rows) * cols;
+
+        o->qt.ensure(cap * nt * sizeof(float2), "hipMalloc(EMI queue)");
+        o->qpix.ensure(cap * sizeof(int), "hipMalloc(EMI queue pixels)");
+        o->qn.ensure(sizeof(int), "hipMalloc(EMI queue count)");
+    }
     for (int r0 = 0; r0 < rows; r0 += chunk) {
         const int nr = std::min(chunk, rows - r0);
 
         mark(2, true);
+        if (queue)
+            check(hipMemsetAsync(o->qn.p, 0, sizeof(int), st), "hipMemset(queue)");
         check(launch_link(kp, (const float2 *)slc,
                           (const uint2 *)o->hl.as<uint32_t>(),
                           o->mask.as<uint32_t>(), (const int *)count,
                           (float *)coh, (float *)pairs, r0, nr, o->precision,
-                          st),
+                          queue ? o->qt.as<float2>() : nullptr,
+                          queue ? o->qpix.as<int>() : nullptr,
+                          queue ? o->qn.as<int>() : nullptr, st),
               "link kernel");
+        if (queue) {
+            int qcount = 0;
+
+            check(hipMemcpyAsync(&qcount, o->qn.p, sizeof(int),
+                                 hipMemcpyDeviceToHost, st),
+                  "hipMemcpy(queue count)");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            if ((size_t)qcount > (size_t)nr * cols)
+                fail("EMI queue of %d pixels for a band of %zu", qcount,
+                     (size_t)nr * cols);
+            check(launch_emi_queue(kp, o->qt.as<float2>(), o->qpix.as<int>(),
+                                   qcount, (float *)coh, st),
+                  "EMI queue kernel");
+        }
         mark(2, false);
     }
     if (timing) {
@@ -854,7 +889,8 @@ extern "C" void ocl_close(struct ocl *o)
         DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
                           &o->count, &o->pairs, &o->ampT,  &o->lstat,
                           &o->mask, &o->adtab, &o->hl,
-                          &o->absmax, &o->grid, &o->rin[0], &o->rin[1],
+                          &o->absmax, &o->qt, &o->qpix, &o->qn,
+                          &o->grid, &o->rin[0], &o->rin[1],
                           &o->rhgt[0], &o->rhgt[1], &o->rout[0], &o->rout[1],
                           &o->rslc, &o->ramp, &o->rvalid};
         if (o->copy)
@@ -887,15 +923,31 @@ extern "C" void ocl_close(struct ocl *o)
 #define TCOH_EIG_CASE(NN)                                                    \
     case NN:                                                                 \
         return launch_link_n<NN>(kp, slc, hl, mask, count, coh, pairs, row0, \
-                                 nrows, precision, s);
+                                 nrows, precision, qt, qpix, qn, s);
 
 static hipError_t launch_link(const KParams &kp, const float2 *slc,
                               const uint2 *hl, const uint32_t *mask,
                               const int *count, float *coh, float *pairs,
-                              int row0, int nrows, int precision, hipStream_t s)
+                              int row0, int nrows, int precision, float2 *qt,
+                              int *qpix, int *qn, hipStream_t s)
 {
     switch (kp.n) {
         TCOH_EIG_FOR_EACH(TCOH_EIG_CASE)
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+#define TCOH_QUEUE_CASE(NN)                                                  \
+    case NN:                                                                 \
+        return launch_emi_queue_n<NN>(kp, qt, qpix, qcount, coh, s);
+
+static hipError_t launch_emi_queue(const KParams &kp, const float2 *qt,
+                                   const int *qpix, int qcount, float *coh,
+                                   hipStream_t s)
+{
+    switch (kp.n) {
+        TCOH_EIG_FOR_EACH(TCOH_QUEUE_CASE)
     default:
         return hipErrorInvalidValue;
     }

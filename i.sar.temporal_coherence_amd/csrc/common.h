@@ -411,11 +411,18 @@ struct StripArgs {
 hipError_t launch_strip(const StripArgs &sa, hipStream_t s);
 /* Fused covariance + linking of output rows row0 .. row0+nrows-1
    (precision: 0 plain, 1 split FP16, 2 split BF16), one instantiation
-   per stack depth N. */
+   per stack depth N. With EMI and a queue (qt non-null, depths up to 32),
+   pixels needing the Jacobi pseudo-inverse are appended to qt / qpix
+   (count in *qn) instead of linked. */
 template <int N>
 hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
                          const uint32_t *mask, const int *count, float *coh,
                          float *pairs, int row0, int nrows, int precision,
-                         hipStream_t s);
+                         float2 *qt, int *qpix, int *qn, hipStream_t s);
+/* EMI pixels deferred by launch_link_n (queue qt, qpix of qcount entries),
+   PPW to a wavefront; depths up to 32. */
+template <int N>
+hipError_t launch_emi_queue_n(const KParams &kp, const float2 *qt, const int *qpix,
+                              int qcount, float *coh, hipStream_t s);
 
 #endif

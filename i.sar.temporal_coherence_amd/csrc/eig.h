@@ -703,10 +703,14 @@ __device__ __forceinline__ float2 load_t(const float2 *__restrict__ tp, int r,
     return make_float2(t.x, -t.y);
 }
 
-/* |T|^-1 into inv (owned columns) for EMI (tcoh.cl:657-687). */
+/* |T|^-1 into inv (owned columns) for EMI (tcoh.cl:657-687). Returns
+   false, with inv untouched, when the Cholesky pivot test fails and the
+   caller defers the Jacobi pseudo-inverse (fallback false) to a queue of
+   such pixels processed four to a wavefront. */
 template <int N, int G, int CPL>
-__device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool est,
-                            EmiLds<N> &em, int lg, float (&inv)[CPL][N])
+__device__ __forceinline__ bool abs_inverse(const float2 *__restrict__ tp, bool est,
+                            EmiLds<N> &em, int lg, float (&inv)[CPL][N],
+                            bool fallback)
 {
     bool ok = true;
 
@@ -776,7 +780,7 @@ __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool 
         }
         wave_sync();
         if (ok)
-            return;
+            return true;
     }
     else {
     double g[CPL][N];
@@ -899,8 +903,10 @@ __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool 
     }
     wave_sync();
     if (ok)
-        return;
+        return true;
     }
+    if (!fallback)
+        return false;
 
     /* Not safely positive definite: pseudo-inverse from a Jacobi
        eigendecomposition of |T| in FP32 with the reference's rotations,
@@ -1155,6 +1161,7 @@ __device__ __forceinline__ void abs_inverse(const float2 *__restrict__ tp, bool 
             inv[m][r] = sm[m];
     }
     wave_sync();
+    return true;
 }
 
 #endif

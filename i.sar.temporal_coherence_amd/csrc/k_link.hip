@@ -37,6 +37,14 @@ template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
 /* cov: 0 plain FP32, 1 split FP16 MFMA, 2 split BF16 MFMA (uniform per
    launch: one code object serves the three forms, so the linking code is
    compiled once per depth and estimator). */
+/* EMI pixels whose |T| fails the Cholesky pivot test, deferred to
+   k_emi_queue: T (NT float2 each), the output pixel, and the count. */
+struct EmiQueue {
+    float2 *t;
+    int *pix;
+    int *n;
+};
+
 template <int N, int EMI>
 __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
@@ -46,7 +54,8 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
                                                float *__restrict__ coh,
                                                float *__restrict__ pairs,
                                                int row0, int nrows,
-                                               int scratch_bytes, int cov)
+                                               int scratch_bytes, int cov,
+                                               EmiQueue q)
 {
     typedef LinkShape<N> S;
     constexpr int G = S::G, PPW = S::PPW, NT = S::NT, NB = S::NB;
@@ -77,19 +86,100 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
                                      scratch, tl, pairs);
     wave_sync();
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(scratch);
+    /* With a queue, a group whose |T| needs the Jacobi pseudo-inverse
+       leaves it to k_emi_queue, where every group of a wavefront needs it:
+       here one such group would hold up the wavefront's other three. */
+    const bool defer = EMI && q.t != nullptr;
+    bool deferred;
     const float gam = link_group<N, G, EMI>(kp, tl + gi * NT, est, lg, sh[gi],
-                                            ems[EMI ? gi : 0]);
+                                            ems[EMI ? gi : 0], !defer, deferred);
+
+    if (defer) {
+        const unsigned long long b = __ballot(lg == 0 && deferred);
+
+        if (b) {
+            int base = 0;
+
+            if (threadIdx.x == 0)
+                base = atomicAdd(q.n, __popcll(b));
+            base = __shfl(base, 0);
+            if (deferred) {
+                const int slot =
+                    base + __popcll(b & ((1ull << (gi * G)) - 1ull));
+                float2 *dst = q.t + (size_t)slot * NT;
+
+                for (int t = lg; t < NT; t += G)
+                    dst[t] = tl[gi * NT + t];
+                if (lg == 0)
+                    q.pix[slot] = p;
+            }
+        }
+    }
+    if (live && lg == 0 && !deferred)
+        coh[p] = gam;
+}
+
+/* The deferred EMI pixels, PPW to a wavefront: T from the queue, the
+   Jacobi pseudo-inverse of |T| (the Cholesky test fails again, as it did in
+   k_link), then the linking as in k_link. */
+template <int N>
+__global__ __launch_bounds__(64, 1) void k_emi_queue(KParams kp,
+                                                     const float2 *__restrict__ qt,
+                                                     const int *__restrict__ qpix,
+                                                     int qcount,
+                                                     float *__restrict__ coh)
+{
+    typedef LinkShape<N> S;
+    constexpr int G = S::G, PPW = S::PPW, NT = S::NT;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    float2 *tl = reinterpret_cast<float2 *>(lds);
+    EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(lds + S::TL_BYTES);
+    EmiLds<N> *ems = reinterpret_cast<EmiLds<N> *>(lds + S::TL_BYTES + S::EIG_BYTES);
+    const int gi = threadIdx.x / G, lg = threadIdx.x % G;
+    const int slot = blockIdx.x * PPW + gi;
+    const bool live = slot < qcount;
+
+    if (live)
+        for (int t = lg; t < NT; t += G)
+            tl[gi * NT + t] = qt[(size_t)slot * NT + t];
+    wave_sync();
+    bool deferred;
+    const float gam = link_group<N, G, 1>(kp, tl + gi * NT, live, lg, sh[gi], ems[gi],
+                                          true, deferred);
 
     if (live && lg == 0)
-        coh[p] = gam;
+        coh[qpix[slot]] = gam;
+}
+
+template <int N>
+hipError_t launch_emi_queue_n(const KParams &kp, const float2 *qt, const int *qpix,
+                              int qcount, float *coh, hipStream_t s)
+{
+    typedef LinkShape<N> S;
+    const size_t lds = (size_t)S::TL_BYTES + S::EIG_BYTES + S::EMI_BYTES;
+
+    if (qcount <= 0)
+        return hipSuccess;
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&k_emi_queue<N>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess)
+            return e;
+    }
+    hipLaunchKernelGGL(k_emi_queue<N>, dim3((qcount + S::PPW - 1) / S::PPW),
+                       dim3(WAVE), lds, s, kp, qt, qpix, qcount, coh);
+    return hipGetLastError();
 }
 
 template <int N>
 hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
                          const uint32_t *mask, const int *count, float *coh,
                          float *pairs, int row0, int nrows, int precision,
-                         hipStream_t s)
+                         float2 *qt, int *qpix, int *qn, hipStream_t s)
 {
+    const EmiQueue q = {qt, qpix, qn};
+
     typedef LinkShape<N> S;
     const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW));
     const int scratch = link_scratch_bytes<N>(kp, precision);
@@ -106,7 +196,7 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
         }                                                                       \
         hipLaunchKernelGGL((k_link<N, EMI>), grid, dim3(WAVE), lds, s, kp, slc, \
                            hl, mask, count, coh, pairs, row0, nrows, scratch,   \
-                           precision);                                          \
+                           precision, q);                                       \
     } while (0)
     if (kp.emi)
         TCOH_LINK_LAUNCH(1);
@@ -121,4 +211,8 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
 template hipError_t launch_link_n<TCOH_EIG_N>(const KParams &, const float2 *,
                                               const uint2 *, const uint32_t *,
                                               const int *, float *, float *, int,
-                                              int, int, hipStream_t);
+                                              int, int, float2 *, int *, int *,
+                                              hipStream_t);
+template hipError_t launch_emi_queue_n<TCOH_EIG_N>(const KParams &, const float2 *,
+                                                   const int *, int, float *,
+                                                   hipStream_t);

@@ -9,13 +9,18 @@
 #include "eig.h"
 
 /* gamma of the group's pixel, NaN when it is not estimated or (with the
-   bias correction) its dates are not linked. */
+   bias correction) its dates are not linked. EMI with fallback false: a
+   pixel whose |T| fails the Cholesky pivot test is not linked here
+   (deferred set, the result is meaningless) and goes to the fallback
+   queue. */
 template <int N, int G, int EMI>
 __device__ __forceinline__ float link_group(const KParams &kp,
                                             const float2 *tp, bool est,
                                             int lg, EigLds<N> &sh,
-                                            EmiLds<N> &em)
+                                            EmiLds<N> &em, bool fallback,
+                                            bool &deferred)
 {
+    deferred = false;
     constexpr int CPL = (N + G - 1) / G;
     float2 A[CPL][N];
 
@@ -84,7 +89,14 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     else {
         float inv[CPL][N];
 
-        abs_inverse<N, G, CPL>(tp, est, em, lg, inv);
+        if (!abs_inverse<N, G, CPL>(tp, est, em, lg, inv, fallback)) {
+            deferred = true;
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+#pragma unroll
+                for (int r = 0; r < N; r++)
+                    inv[m][r] = 0.0f;
+        }
         /* M = |T|^-1 o T, reloaded so that T need not stay in registers
            through the inversion. */
 #pragma unroll
