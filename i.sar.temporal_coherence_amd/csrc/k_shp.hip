@@ -111,8 +111,12 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                                            uint32_t *__restrict__ mask,
                                            int *__restrict__ count)
 {
-    /* AD walks the centre sample with a data-dependent index. */
+    /* AD walks the centre's and the candidate's samples with data-dependent
+       indices: both sit in LDS, one column per lane (the candidate's loaded
+       in one burst of independent loads), so that a step of the walk waits
+       on LDS rather than on a global load. */
     __shared__ float acol[TEST == 1 ? NB : 1][WAVE];
+    __shared__ float bcol[TEST == 1 ? NB : 1][WAVE];
     const int n = kp.n;
     int y, tx;
 
@@ -185,6 +189,21 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                 if (TEST == 1) {
                     int ia = 0, ib = 0, m = 0;
                     float s = 0.0f;
+                    float b[NB];
+                    int idx = cand;
+
+#pragma unroll
+                    for (int i = 0; i < NB; i++) {
+                        b[i] = ampT[idx];
+                        idx += npv;
+                    }
+#pragma unroll
+                    for (int i = 0; i < NB; i++)
+                        if (i < n)
+                            bcol[i][threadIdx.x] = b[i];
+                    /* the current sample of each side, reloaded only for
+                       the side that advances */
+                    float va = acol[0][threadIdx.x], vb = bcol[0][threadIdx.x];
 
                     for (int j = 1; j < 2 * n; j++) {
                         bool take_a;
@@ -194,11 +213,14 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                         else if (ib >= n)
                             take_a = true;
                         else
-                            take_a = acol[ia][threadIdx.x] <=
-                                     ampT[(size_t)ib * kp.npad + cand];
+                            take_a = va <= vb;
                         m += take_a;
                         ia += take_a;
                         ib += !take_a;
+                        if (take_a)
+                            va = acol[min(ia, n - 1)][threadIdx.x];
+                        else
+                            vb = bcol[min(ib, n - 1)][threadIdx.x];
                         s = s + adtab[j * (n + 1) + m];
                     }
                     /* plain operators: contraction is off in this file
