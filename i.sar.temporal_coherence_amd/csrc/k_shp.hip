@@ -39,29 +39,49 @@ __device__ __forceinline__ float logf_rounded(float x)
 }
 
 /* ampT[k][pix] = amp[pix][k]; lstat[pix] = (mean, var) of log amplitudes
-   of valid pixels when the t-test is used. */
-__global__ __launch_bounds__(256) void k_prep(KParams kp,
-                                             const float *__restrict__ amp,
-                                             float *__restrict__ ampT,
-                                             const unsigned char *__restrict__ valid,
-                                             float2 *__restrict__ lstat)
-{
-    const int pix = blockIdx.x * 256 + threadIdx.x;
-    const int n = kp.n;
+   of valid pixels when the t-test is used. A block stages its PREP_PX
+   pixels' amplitudes (contiguous in amp) in LDS with coalesced reads, then
+   writes the date planes; a row of the tile is padded to n + 1 words so
+   that the per-pixel reads do not collide in LDS banks. */
+#define PREP_PX 128
 
-    if (pix >= kp.npad)
+__global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
+                                                 const float *__restrict__ amp,
+                                                 float *__restrict__ ampT,
+                                                 const unsigned char *__restrict__ valid,
+                                                 float2 *__restrict__ lstat)
+{
+    __shared__ float tile[PREP_PX * (64 + 1)]; /* up to 64 dates */
+    const int n = kp.n, n1 = n + 1;
+    const int p0 = blockIdx.x * PREP_PX;
+    const int np = min(PREP_PX, kp.npad - p0);
+    const float *src = amp + (size_t)p0 * n;
+    const float inv_n = 1.0f / (float)n;
+
+    /* t < 128 * 64: (t + 0.5) / n is at least 0.5 / n away from an
+       integer, far more than the float product's error */
+    for (int t = threadIdx.x; t < np * n; t += PREP_PX) {
+        const int pp = (int)(((float)t + 0.5f) * inv_n);
+
+        tile[pp * n1 + (t - pp * n)] = src[t];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x >= np)
         return;
+    const int pix = p0 + threadIdx.x;
+    const float *a = tile + threadIdx.x * n1;
+
     for (int k = 0; k < shp_bucket(n); k++)
-        ampT[(size_t)k * kp.npad + pix] = k < n ? amp[(size_t)pix * n + k] : 0.0f;
+        ampT[(size_t)k * kp.npad + pix] = k < n ? a[k] : 0.0f;
     if (kp.test == 2) {
         float mean = 0.0f, var = 0.0f;
 
         if (valid[pix]) {
             for (int k = 0; k < n; k++)
-                mean += logf_rounded(amp[(size_t)pix * n + k]);
+                mean += logf_rounded(a[k]);
             mean /= n;
             for (int k = 0; k < n; k++) {
-                const float d = logf_rounded(amp[(size_t)pix * n + k]) - mean;
+                const float d = logf_rounded(a[k]) - mean;
 
                 var += d * d;
             }
@@ -255,8 +275,8 @@ hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
                        const unsigned char *valid, float2 *lstat,
                        hipStream_t s)
 {
-    hipLaunchKernelGGL(k_prep, dim3((kp.npad + 255) / 256), dim3(256), 0, s,
-                       kp, amp, ampT, valid, lstat);
+    hipLaunchKernelGGL(k_prep, dim3((kp.npad + PREP_PX - 1) / PREP_PX), dim3(PREP_PX),
+                       0, s, kp, amp, ampT, valid, lstat);
     return hipGetLastError();
 }
 

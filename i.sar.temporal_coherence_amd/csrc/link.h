@@ -184,14 +184,21 @@ __device__ __forceinline__ float link_group(const KParams &kp,
         if (!est || pp * G >= NPAIR || c1 >= NPAIR)
             continue;
         const float2 z1 = sh.z[c1], z2 = sh.z[c2];
+        /* T_{i,col} at upper_index(N, i, col), stepped incrementally: col - 1
+           at i = 0, then + (N - 2 - i) per row */
+        int addr = c1 - 1, stp = N - 2, i = 0;
 
 #pragma unroll 4
         for (int t = 0; t < N - 1; t++) {
             const bool first = t < c1;
-            const int col = first ? c1 : c2;
-            const int i = first ? t : t - c1;
+
+            if (t == c1) {
+                addr = c2 - 1;
+                stp = N - 2;
+                i = 0;
+            }
             const float2 zc = first ? z1 : z2;
-            const float2 tv = tp[upper_index(N, i, col)];
+            const float2 tv = tp[addr];
             const float m2 = fmaf(tv.x, tv.x, tv.y * tv.y);
             const float rm = __builtin_amdgcn_rsqf(m2);
             const float ur = m2 > 0.0f ? tv.x * rm : 1.0f;
@@ -201,6 +208,9 @@ __device__ __forceinline__ float link_group(const KParams &kp,
 
             gr = fmaf(wt, fmaf(ur, w.x, -ui * w.y), gr);
             gim = fmaf(wt, fmaf(ur, w.y, ui * w.x), gim);
+            addr += stp;
+            stp -= 1;
+            i += 1;
         }
     }
     gr = gsum<G>(gr);
