@@ -56,6 +56,19 @@ class Settings(C.Structure):
     ]
 
 
+class Utc(C.Structure):
+    """struct utc, local_proto.h:15-18."""
+
+    _fields_ = [
+        ("year", C.c_int),
+        ("month", C.c_int),
+        ("day", C.c_int),
+        ("hour", C.c_int),
+        ("minute", C.c_int),
+        ("second", C.c_double),
+    ]
+
+
 class Orbit(C.Structure):
     """struct orbit, local_proto.h:52-57."""
 
@@ -142,6 +155,30 @@ def lib():
     L.geometry_phases.argtypes = [C.POINTER(Geometry), C.c_int, C.c_int,
                                   C.c_double, vp]
     L.geometry_free.argtypes = [C.POINTER(Geometry)]
+    cp = C.c_char_p
+    L.tcoh_utc_seconds.argtypes = [C.POINTER(Utc)]
+    L.tcoh_utc_seconds.restype = C.c_double
+    L.tcoh_utc_from_seconds.argtypes = [C.c_double, C.POINTER(Utc)]
+    L.tcoh_parse_utc.argtypes = [cp, C.POINTER(C.c_double)]
+    L.tcoh_json_get.argtypes = [cp, cp]
+    L.tcoh_json_get.restype = vp
+    L.tcoh_json_get_number.argtypes = [cp, cp]
+    L.tcoh_json_get_number.restype = C.c_double
+    L.tcoh_json_array_length.argtypes = [cp, cp]
+    L.tcoh_free.argtypes = [vp]
+    L.tcoh_acquisition_span.argtypes = [cp, cp, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
+    L.tcoh_orbit_from_annotation.argtypes = [cp, cp, C.POINTER(Orbit)]
+    L.tcoh_orbit_from_eof.argtypes = [cp, C.c_double, C.c_double, cp,
+                                      C.POINTER(Orbit)]
+    L.tcoh_orbit_pick_file.argtypes = [cp, cp, C.c_double, C.c_double, cp,
+                                       C.c_size_t]
+    L.tcoh_orbit_from_file.argtypes = [cp, cp, cp, cp, C.c_int, C.POINTER(Orbit)]
+    L.tcoh_orbit_best.argtypes = [cp, cp, cp, C.POINTER(Orbit)]
+    L.tcoh_orbit_free.argtypes = [C.POINTER(Orbit)]
+    L.tcoh_scene_from_metadata.argtypes = [C.POINTER(cp), C.POINTER(cp), C.c_int,
+                                           C.c_int, C.c_double, C.c_double,
+                                           C.c_int, C.c_int, C.POINTER(Scene)]
     # The HIP runtime the library links (found through the library's own
     # dependencies), for DeviceBuffer.
     L.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
@@ -474,3 +511,137 @@ def list_devices():
     """ocl_list_devices (ocl.c:105-136), printed to stdout."""
     lib().ocl_list_devices()
     _raise_if_error()
+
+
+# ---- orbit ingestion (orbit.c, jsonget.c, epoch.c; host only) ----------
+
+
+def _b(text):
+    return None if text is None else (text.encode() if isinstance(text, str) else text)
+
+
+def utc_seconds(year, month, day, hour=0, minute=0, second=0.0):
+    """tcoh_utc_seconds (epoch.c:28-32): seconds since 1970 UTC."""
+    return lib().tcoh_utc_seconds(C.byref(Utc(year, month, day, hour, minute, second)))
+
+
+def utc_from_seconds(sec):
+    """tcoh_utc_from_seconds (epoch.c:34-52): (year, month, day, hour,
+    minute, second)."""
+    u = Utc()
+    lib().tcoh_utc_from_seconds(sec, C.byref(u))
+    return (u.year, u.month, u.day, u.hour, u.minute, u.second)
+
+
+def parse_utc(text):
+    """tcoh_parse_utc (orbit.c:47-56): seconds, or None if unreadable."""
+    x = C.c_double()
+    return x.value if lib().tcoh_parse_utc(_b(text), C.byref(x)) else None
+
+
+def json_get(text, path):
+    """tcoh_json_get (jsonget.c:171-190): str or None."""
+    L = lib()
+    p = L.tcoh_json_get(_b(text), _b(path))
+    if not p:
+        return None
+    try:
+        return C.string_at(p).decode()
+    finally:
+        L.tcoh_free(p)
+
+
+def json_get_number(text, path):
+    return lib().tcoh_json_get_number(_b(text), _b(path))
+
+
+def json_array_length(text, path):
+    return lib().tcoh_json_array_length(_b(text), _b(path))
+
+
+def acquisition_span(meta, name="date"):
+    """tcoh_acquisition_span (orbit.c:59-71): (start, stop) seconds."""
+    a, b = C.c_double(), C.c_double()
+    rc = lib().tcoh_acquisition_span(_b(meta), _b(name), C.byref(a), C.byref(b))
+    _raise_if_error()
+    assert rc == 0
+    return a.value, b.value
+
+
+def _take_orbit(o):
+    """Copy a library-filled struct orbit to (t, pos, vel, source) arrays and
+    release it."""
+    n = o.n
+    if n == 0:
+        t, pos, vel = np.zeros(0), np.zeros((0, 3)), np.zeros((0, 3))
+    else:
+        t = np.ctypeslib.as_array(o.t, shape=(n,)).copy()
+        pos = np.ctypeslib.as_array(o.pos, shape=(3 * n,)).copy().reshape(n, 3)
+        vel = np.ctypeslib.as_array(o.vel, shape=(3 * n,)).copy().reshape(n, 3)
+    source = o.source.decode()
+    lib().tcoh_orbit_free(C.byref(o))
+    return t, pos, vel, source
+
+
+def orbit_from_annotation(meta, name="date"):
+    """tcoh_orbit_from_annotation (orbit.c:81-111)."""
+    o = Orbit()
+    rc = lib().tcoh_orbit_from_annotation(_b(meta), _b(name), C.byref(o))
+    _raise_if_error()
+    assert rc == 0
+    return _take_orbit(o)
+
+
+def orbit_from_eof(text, t0, t1, source="POEORB"):
+    """tcoh_orbit_from_eof (orbit.c:175-203)."""
+    o = Orbit()
+    rc = lib().tcoh_orbit_from_eof(_b(text), t0, t1, _b(source), C.byref(o))
+    _raise_if_error()
+    assert rc == 0
+    return _take_orbit(o)
+
+
+def orbit_pick_file(directory, prefix, t0, t1):
+    """tcoh_orbit_pick_file (orbit.c:114-151): file name or None."""
+    buf = C.create_string_buffer(1024)
+    found = lib().tcoh_orbit_pick_file(_b(str(directory)), _b(prefix), t0, t1, buf,
+                                       len(buf))
+    return buf.value.decode() if found else None
+
+
+def orbit_from_file(meta, kind, cache_dir, required=False, name="date"):
+    """tcoh_orbit_from_file (orbit.c:205-301, local files): the orbit, or
+    None when no file covers the acquisition and required is false."""
+    o = Orbit()
+    rc = lib().tcoh_orbit_from_file(_b(meta), _b(name), _b(kind), _b(str(cache_dir)),
+                                    int(required), C.byref(o))
+    _raise_if_error()
+    return _take_orbit(o) if rc == 1 else None
+
+
+def orbit_best(meta, cache_dir=None, name="date"):
+    """tcoh_orbit_best (orbit.c:304-318)."""
+    o = Orbit()
+    rc = lib().tcoh_orbit_best(_b(meta), _b(name),
+                               None if cache_dir is None else _b(str(cache_dir)),
+                               C.byref(o))
+    _raise_if_error()
+    assert rc == 0
+    return _take_orbit(o)
+
+
+def scene_from_metadata(metas, ref, line0, sample0, rows, cols, names=None):
+    """tcoh_scene_from_metadata (geometry.c:165-191): the scene dict that
+    geometry_grid takes."""
+    n = len(metas)
+    names = names or [f"date{k + 1}" for k in range(n)]
+    m = (C.c_char_p * n)(*[_b(x) for x in metas])
+    nm = (C.c_char_p * n)(*[_b(x) for x in names])
+    sc = Scene()
+    rc = lib().tcoh_scene_from_metadata(m, nm, n, ref, line0, sample0, rows, cols,
+                                        C.byref(sc))
+    _raise_if_error()
+    assert rc == 0
+    return dict(t_first=sc.t_first, dt=sc.dt, tau0=sc.tau0, fs=sc.fs,
+                **{"lambda": sc.lambda_}, line0=sc.line0, sample0=sc.sample0,
+                rows=sc.rows, cols=sc.cols)

@@ -332,6 +332,12 @@ extern "C" void tcoh_set_error_mode(int mode) { g_error_mode = mode; }
 
 extern "C" const char *tcoh_last_error(void) { return g_last_error.c_str(); }
 
+/* For the host-only C sources of the library (orbit ingestion): report a
+   failure under the error mode, and clear the message at an entry point. */
+extern "C" void tcoh_report_error(const char *msg) { report(msg); }
+
+extern "C" void tcoh_clear_error(void) { g_last_error.clear(); }
+
 /* q(j, m) = (float)(2Nm - jN)^2 / (float)(j(2N - j)) with the two roundings
    of tcoh.cl:71-72. */
 #pragma clang fp contract(off)

@@ -189,3 +189,85 @@ def geometry_case(n=5, rows=300, cols=500, ref=None, seed=0, days=12.0):
                  fs=64.34523812571427e6, **{"lambda": LIGHT / 5.405e9},
                  line0=0.0, sample0=0.0, rows=rows, cols=cols)
     return circular_orbits(t0s, base), scene, ref
+
+
+# ---- the documents orbit ingestion reads -------------------------------
+# Layouts of a date's description.json (the r.in.s1slc sections the
+# geometry reads) and of a Sentinel-1 POEORB/RESORB EOF file, after
+# tests/geometry_model.py:145-220.
+
+
+def iso_utc(sec):
+    """ISO 8601 text with microseconds of seconds since 1970 UTC."""
+    import datetime as _dt
+
+    t = _dt.datetime(1970, 1, 1) + _dt.timedelta(seconds=float(sec))
+    return t.isoformat(timespec="microseconds")
+
+
+def s1_metadata(orbit, scene, first_line, mission="S1A"):
+    """description.json text of one date: swath (mission, start/stop, radar
+    constants, state vectors) and raster_geometry sections. orbit is
+    (t, pos, vel); first_line the date's first line time (s since 1970)."""
+    import json as _json
+
+    t, pos, vel = orbit
+    start = first_line
+    stop = first_line + (scene["rows"] - 1) * scene["dt"]
+    svs = [{"time": iso_utc(t[i]), "frame": "Earth Fixed",
+            "position": [float(x) for x in pos[i]],
+            "velocity": [float(x) for x in vel[i]]} for i in range(len(t))]
+    doc = {
+        "swath": {
+            "mission": mission,
+            "start_time": iso_utc(start),
+            "stop_time": iso_utc(stop),
+            "wavelength": scene["lambda"],
+            "range_sampling_rate": scene["fs"],
+            "terrain_height": 0.0,
+            "orbit_state_vectors": svs,
+        },
+        "raster_geometry": {
+            "first_line_time": iso_utc(start),
+            "last_line_time": iso_utc(stop),
+            "azimuth_time_interval": scene["dt"],
+            "slant_range_time_first_sample": scene["tau0"],
+        },
+    }
+    return _json.dumps(doc, indent=2)
+
+
+def eof_text(orbit):
+    """EOF document (Earth Explorer file) with one <OSV> per state vector,
+    coordinates printed with 6 decimals as ESA does."""
+    t, pos, vel = orbit
+    recs = []
+    for i in range(len(t)):
+        u = iso_utc(t[i])
+        recs.append(
+            "      <OSV>\n"
+            f"        <TAI>TAI={u}</TAI>\n"
+            f"        <UTC>UTC={u}</UTC>\n"
+            f"        <UT1>UT1={u}</UT1>\n"
+            "        <Absolute_Orbit>+46742</Absolute_Orbit>\n"
+            + "".join(f'        <{tag} unit="{unit}">{val:.6f}</{tag}>\n'
+                      for tag, unit, val in zip(
+                          ("X", "Y", "Z", "VX", "VY", "VZ"),
+                          ("m",) * 3 + ("m/s",) * 3,
+                          list(pos[i]) + list(vel[i])))
+            + "        <Quality>NOMINAL</Quality>\n"
+            "      </OSV>\n")
+    return ('<?xml version="1.0" ?>\n<Earth_Explorer_File>\n  <Data_Block type="xml">\n'
+            f'    <List_of_OSVs count="{len(recs)}">\n' + "".join(recs)
+            + "    </List_of_OSVs>\n  </Data_Block>\n</Earth_Explorer_File>\n")
+
+
+def eof_name(v0, v1, kind="POEORB", mission="S1A", produced="20990101T000000"):
+    """<mission>_OPER_AUX_<kind>_OPOD_<production>_V<v0>_<v1>.EOF"""
+    import datetime as _dt
+
+    def tag(sec):
+        t = _dt.datetime(1970, 1, 1) + _dt.timedelta(seconds=float(sec))
+        return t.strftime("%Y%m%dT%H%M%S")
+
+    return f"{mission}_OPER_AUX_{kind}_OPOD_{produced}_V{tag(v0)}_{tag(v1)}.EOF"

@@ -49,6 +49,12 @@ struct settings {
     const char *orbit;           /* orbit source, NULL without phase */
 };
 
+/* local_proto.h:15-18, field for field. */
+struct utc {
+    int year, month, day, hour, minute;
+    double second;
+};
+
 /* local_proto.h:52-57, field for field. */
 struct orbit {
     int n;
@@ -68,6 +74,12 @@ struct geometry {
 };
 
 #endif /* TCOH_HAVE_REFERENCE_TYPES */
+
+/* Visible at file scope either way, for the prototypes below. */
+struct settings;
+struct utc;
+struct orbit;
+struct geometry;
 
 /* stats.c:61-95: KS_MAX, AD_NORM, AD_SIGMA, AD_CRIT, TLOG_CRIT from
    s->ndates, s->alpha and s->test. */
@@ -128,6 +140,82 @@ void geometry_phases(const struct geometry *g, int row, int col, double h,
 
 /* geometry.c:293-296 */
 void geometry_free(struct geometry *g);
+
+/* ---- Orbit ingestion (host; orbit.c, jsonget.c, epoch.c) ----
+   The reference reads a date's metadata through struct epoch, a GRASS
+   type; these take the date's description.json text (meta) and its name
+   (for messages) instead. Returns: 0 or 1 as stated on success, -1 after a
+   failure that error mode 1 turned into a return. */
+
+/* epoch.c:28-32: seconds since 1970-01-01T00:00:00 UTC. */
+double tcoh_utc_seconds(const struct utc *t);
+
+/* epoch.c:34-52 */
+void tcoh_utc_from_seconds(double sec, struct utc *t);
+
+/* orbit.c:47-56: "YYYY-MM-DDThh:mm:ss[.ffffff]" to seconds; 1 when read,
+   0 otherwise. */
+int tcoh_parse_utc(const char *text, double *sec);
+
+/* jsonget.c:171-190: value at a dotted path (a numeric component indexes an
+   array) as a new string, to be released with tcoh_free; NULL if the path
+   is absent, null, an object or an array. */
+char *tcoh_json_get(const char *text, const char *path);
+
+/* jsonget.c:192-205: the number at path, NAN if absent or not a number. */
+double tcoh_json_get_number(const char *text, const char *path);
+
+/* jsonget.c:207-225: elements of the array at path, -1 if not an array. */
+int tcoh_json_array_length(const char *text, const char *path);
+
+void tcoh_free(void *p);
+
+/* orbit.c:59-71 with epoch.c:70-85's fallback: swath.start_time and
+   stop_time, else the first and last line times. 0 on success. */
+int tcoh_acquisition_span(const char *meta, const char *name, double *start,
+                          double *stop);
+
+/* orbit.c:81-111: swath.orbit_state_vectors of the metadata; fails unless
+   8 or more vectors cover the acquisition. 0 on success; o is released
+   with tcoh_orbit_free. */
+int tcoh_orbit_from_annotation(const char *meta, const char *name,
+                               struct orbit *o);
+
+/* orbit.c:175-203: the <OSV> records of an EOF document with UTC time in
+   [t0, t1]; source names the orbit in messages. 0 on success. */
+int tcoh_orbit_from_eof(const char *text, double t0, double t1,
+                        const char *source, struct orbit *o);
+
+/* orbit.c:114-151: the .EOF file of dir whose name starts with prefix and
+   whose validity _V<start>_<stop> covers [t0, t1], the greatest name when
+   several do (the latest production). 1 when found (name in out). */
+int tcoh_orbit_pick_file(const char *dir, const char *prefix, double t0,
+                         double t1, char *out, size_t out_size);
+
+/* orbit.c:205-301 for a local cache_dir: the <mission>_OPER_AUX_<kind>_OPOD_
+   file (kind POEORB or RESORB) covering the acquisition with 120 s of
+   margin, its state vectors within that margin. 1 when read, 0 when no file
+   covers it and required is 0. The reference's download from the ESA
+   mirror is not done (no network access is assumed). */
+int tcoh_orbit_from_file(const char *meta, const char *name, const char *kind,
+                         const char *cache_dir, int required, struct orbit *o);
+
+/* orbit.c:304-318: precise, else restituted, else annotation. 0 on
+   success. */
+int tcoh_orbit_best(const char *meta, const char *name, const char *cache_dir,
+                    struct orbit *o);
+
+/* orbit.c:348-353 */
+void tcoh_orbit_free(struct orbit *o);
+
+/* geometry.c:165-191: the scene numbers geometry_init reads from the
+   reference date's metadata, and its check that every date has the same
+   wavelength. line0/sample0 (the first region cell in the reference image)
+   and rows/cols come from the caller's region. 0 on success. */
+int tcoh_scene_from_metadata(const char *const *metas,
+                             const char *const *names, int n, int ref,
+                             double line0, double sample0, int rows, int cols,
+                             struct tcoh_scene *sc);
 
 /* ---- Extensions (no counterpart in the reference) ---- */
 

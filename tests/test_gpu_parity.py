@@ -202,3 +202,37 @@ def test_geometry_grid_matches_oracle(tcoh, cref):
     # under 1 m of relief: one height, the middle (geometry.c:204-213)
     flat, fhs = tcoh.api.geometry_grid(orbits, scene, ref, 100.0, 100.5)
     assert flat.shape[2] == 1 and fhs[0] == 100.25
+
+
+def test_geometry_from_ingested_orbits(tcoh, cref, tmp_path):
+    """The geometry inputs read as main.c would have them: state vectors
+    from POEORB files in a local directory (orbit.c:205-318) and from the
+    annotation (orbit.c:81-111), scene numbers from the reference date's
+    metadata (geometry.c:165-191). The GPU grid on the file orbits equals the
+    FP64 restatement on the same orbits to 1e-6 rad, and the two orbit
+    sources agree to 1e-3 rad (the EOF's 1 um rounding is 4.5e-4 rad at
+    C-band)."""
+    api, syn = tcoh.api, tcoh.synth
+    api.lib().tcoh_set_error_mode(1)
+    orbits, scene, ref = syn.geometry_case(n=4, rows=300, cols=500, seed=4)
+    t0s = [o[0][len(o[0]) // 2] for o in orbits]
+    base = [o[1][0, 1] for o in orbits]
+    wide = syn.circular_orbits(t0s, base, half_span=600.0)
+    metas = []
+    for k in range(len(t0s)):
+        metas.append(syn.s1_metadata(orbits[k], scene, t0s[k]))
+        (tmp_path / syn.eof_name(t0s[k] - 599, t0s[k] + 599)).write_text(
+            syn.eof_text(wide[k]))
+    sc = api.scene_from_metadata(metas, ref, 0.0, 0.0, scene["rows"], scene["cols"])
+    eof = [api.orbit_best(m, tmp_path) for m in metas]
+    assert all(o[3] == "POEORB" for o in eof)
+    eof = [o[:3] for o in eof]
+    ann = [api.orbit_from_annotation(m)[:3] for m in metas]
+    got, hs = api.geometry_grid(eof, sc, ref, 0.0, 600.0)
+    want, ohs, status = cref.geometry_grid(eof, sc, ref, 0.0, 600.0)
+    assert status == 0 and got.shape == want.shape and np.array_equal(hs, ohs)
+    err = np.max(np.abs(got - want))
+    assert err <= 1e-6, "max |d phase| %.3g rad" % err
+    from_ann, _ = api.geometry_grid(ann, sc, ref, 0.0, 600.0)
+    err = np.max(np.abs(got - from_ann))
+    assert err <= 1e-3, "EOF vs annotation: max |d phase| %.3g rad" % err
