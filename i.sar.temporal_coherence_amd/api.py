@@ -179,6 +179,16 @@ def lib():
     L.tcoh_scene_from_metadata.argtypes = [C.POINTER(cp), C.POINTER(cp), C.c_int,
                                            C.c_int, C.c_double, C.c_double,
                                            C.c_int, C.c_int, C.POINTER(Scene)]
+    L.tcoh_pair_names.argtypes = [cp, C.c_int, C.c_int, C.POINTER(Utc),
+                                  C.POINTER(C.c_int), vp, C.c_size_t,
+                                  C.POINTER(C.c_int)]
+    L.tcoh_raw_map.argtypes = [cp, C.c_size_t, C.POINTER(vp)]
+    L.tcoh_raw_unmap.argtypes = [vp, C.c_size_t]
+    L.tcoh_raw_create.argtypes = [cp, cp, cp, C.POINTER(cp), C.c_int, C.c_int,
+                                  C.c_int]
+    L.tcoh_raw_create.restype = vp
+    L.tcoh_raw_put_strip.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.tcoh_raw_close.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_double)]
     # The HIP runtime the library links (found through the library's own
     # dependencies), for DeviceBuffer.
     L.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
@@ -645,3 +655,100 @@ def scene_from_metadata(metas, ref, line0, sample0, rows, cols, names=None):
     return dict(t_first=sc.t_first, dt=sc.dt, tau0=sc.tau0, fs=sc.fs,
                 **{"lambda": sc.lambda_}, line0=sc.line0, sample0=sc.sample0,
                 rows=sc.rows, cols=sc.cols)
+
+
+# ---- the module's data either side of the estimate (main.c; host) --------
+
+
+def pair_names(prefix, n, pairs_all=False, starts=None, has_time=None):
+    """tcoh_pair_names (main.c:602-645): the pair maps' names in ocl_run's
+    pair order, and whether they carry dates. starts: one datetime (or
+    (year, month, day, hour, minute, second)) per date, or None."""
+    npairs = n * (n - 1) // 2 if pairs_all else n - 1
+    stride = len(prefix) + 40
+    buf = C.create_string_buffer(max(npairs, 1) * stride)
+    st = None
+    if starts is not None:
+        st = (Utc * n)()
+        for k, t in enumerate(starts):
+            f = (t.year, t.month, t.day, t.hour, t.minute,
+                 t.second + t.microsecond * 1e-6) if hasattr(t, "year") else tuple(t)
+            st[k] = Utc(*f)
+    ht = None if has_time is None else (C.c_int * n)(*[int(bool(x)) for x in has_time])
+    dated = C.c_int()
+    got = lib().tcoh_pair_names(_b(prefix), n, int(pairs_all), st, ht, buf, stride,
+                                C.byref(dated))
+    _raise_if_error()
+    assert got == npairs
+    names = [buf.raw[p * stride:(p + 1) * stride].split(b"\0", 1)[0].decode()
+             for p in range(npairs)]
+    return names, bool(dated.value)
+
+
+class RawMaps:
+    """tcoh_raw_create / put_strip / close (main.c:796-824, 921-922): the
+    output maps <dir>/<name>.raw + .hdr, strips appended in row order;
+    close() returns (estimated pixels, mean gamma)."""
+
+    def __init__(self, directory, output, rows, cols, shp_count=None, pair_maps=()):
+        self.npairs = len(pair_maps)
+        names = (C.c_char_p * max(self.npairs, 1))(*[_b(x) for x in pair_maps])
+        self.h = lib().tcoh_raw_create(_b(str(directory)), _b(output), _b(shp_count),
+                                       names, self.npairs, rows, cols)
+        _raise_if_error()
+        if not self.h:
+            raise RuntimeError("tcoh_raw_create failed")
+        self.cols = cols
+
+    def put_strip(self, coh, count, pairs=None):
+        coh = np.ascontiguousarray(coh, np.float32).reshape(-1)
+        count = np.ascontiguousarray(count, np.int32).reshape(-1)
+        rows = coh.size // self.cols
+        pr = None if not self.npairs else np.ascontiguousarray(pairs, np.float32)
+        rc = lib().tcoh_raw_put_strip(self.h, rows, _ptr(coh), _ptr(count), _ptr(pr))
+        _raise_if_error()
+        assert rc == 0
+
+    def close(self):
+        est, mean = C.c_long(), C.c_double()
+        h, self.h = self.h, None
+        lib().tcoh_raw_close(h, C.byref(est), C.byref(mean))
+        _raise_if_error()
+        return est.value, mean.value
+
+
+def run_raw_region(backend, path_i, path_q, rows, cols, directory, output,
+                   shp_count=None, pairs_prefix=None, starts=None,
+                   strip_rows=0):
+    """The module's loop without GRASS: the I and Q date planes
+    [N][rows][cols] float32 of two raw files, mapped in place
+    (tcoh_raw_map) and streamed through the GPU (tcoh_run_region); the maps
+    written with the module's nulls and names. Returns (estimated pixels,
+    mean gamma, pair map names)."""
+    s = backend.settings
+    n = s.ndates
+    nbytes = n * rows * cols * 4
+    L = lib()
+    addr = [C.c_void_p(), C.c_void_p()]
+    try:
+        for k, path in enumerate((path_i, path_q)):
+            L.tcoh_raw_map(_b(str(path)), nbytes, C.byref(addr[k]))
+            _raise_if_error()
+        names = []
+        if s.npairs:
+            if pairs_prefix is None:
+                raise ValueError("pairs_prefix is needed with pair maps")
+            names, _ = pair_names(pairs_prefix, n, bool(s.pairs_all), starts)
+        coh = np.empty(rows * cols, np.float32)
+        count = np.empty(rows * cols, np.int32)
+        pairs = np.empty((s.npairs, rows * cols), np.float32) if s.npairs else None
+        L.tcoh_run_region(backend.h, rows, cols, addr[0], addr[1], None, None,
+                          _ptr(coh), _ptr(count), _ptr(pairs), strip_rows)
+        _raise_if_error()
+    finally:
+        for a in addr:
+            L.tcoh_raw_unmap(a, nbytes)
+    maps = RawMaps(directory, output, rows, cols, shp_count, names)
+    maps.put_strip(coh, count, pairs)
+    est, mean = maps.close()
+    return est, mean, names

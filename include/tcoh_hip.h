@@ -208,6 +208,41 @@ int tcoh_orbit_best(const char *meta, const char *name, const char *cache_dir,
 /* orbit.c:348-353 */
 void tcoh_orbit_free(struct orbit *o);
 
+/* ---- The module's data either side of the estimate (host; main.c) ----
+   Raw files hold what GRASS stores for a cell, uncompressed, in native byte
+   order: float32 with null as the all-ones word, int32 with null as
+   INT32_MIN. Extensions: the reference reads and writes GRASS rasters. */
+
+/* main.c:602-645: the names <prefix>_<a>_<b> of the pair maps, in the pair
+   order of ocl_run's output, into names (npairs strings of stride bytes).
+   a and b are the dates' YYYYMMDD, with Thhmmss when two consecutive dates
+   share a day, or their 1-based ranks when any date has no time (starts
+   NULL, or has_time[i] 0; has_time NULL means every date has one). *dated
+   (if not NULL) tells which. Returns the number of pairs. */
+int tcoh_pair_names(const char *prefix, int n, int pairs_all,
+                    const struct utc *starts, const int *has_time,
+                    char *names, size_t stride, int *dated);
+
+/* The first bytes of a raw file mapped read-only (for instance the I or Q
+   planes [N][rows][cols] float32 that tcoh_run_region reads, as the rows
+   read_strip takes from <date>_i / <date>_q, main.c:70-72). 0 on success. */
+int tcoh_raw_map(const char *path, size_t bytes, const void **addr);
+void tcoh_raw_unmap(const void *addr, size_t bytes);
+
+/* main.c:796-824: the output maps <dir>/<name>.raw (+ .hdr) of a region of
+   rows x cols: coherence (output), SHP count (shp_count, or NULL for none)
+   and npairs pair maps. put_strip appends the rows x cols outputs of one
+   ocl_run (count < 0 and NaN become null); close writes the headers and
+   gives main.c:921-922's summary: estimated pixels and their mean. */
+struct tcoh_raw_out;
+struct tcoh_raw_out *tcoh_raw_create(const char *dir, const char *output,
+                                     const char *shp_count,
+                                     const char *const *pair_names, int npairs,
+                                     int rows, int cols);
+int tcoh_raw_put_strip(struct tcoh_raw_out *w, int rows, const float *coh,
+                       const int *count, const float *pairs);
+int tcoh_raw_close(struct tcoh_raw_out *w, long *estimated, double *mean);
+
 /* geometry.c:165-191: the scene numbers geometry_init reads from the
    reference date's metadata, and its check that every date has the same
    wavelength. line0/sample0 (the first region cell in the reference image)

@@ -206,3 +206,39 @@ def test_region_stream_with_geometric_phase(tcoh, cref):
             same_maps(be.run_region(i, q, geometry=g, heights=heights,
                                     strip_rows=strip_rows), want)
     assert want[1][10 * cols + 11] == -1
+
+
+def test_raw_region_writes_the_module_maps(tcoh, tmp_path):
+    """The module's loop without GRASS: raw I/Q date planes mapped in place,
+    streamed through the GPU, written as maps with GRASS's null words, the
+    reference's pair-map names and main.c:921-922's summary; the values are
+    those of the in-memory region run."""
+    import datetime as dt
+
+    api = tcoh.api
+    n, rows, cols = 6, 45, 38
+    i, q = planes(tcoh, n, rows, cols, seed=51)
+    i.tofile(tmp_path / "stack_i.raw")
+    q.tofile(tmp_path / "stack_q.raw")
+    s = tcoh.make_settings(n, window=WINDOW, min_shp=9, pairs="consecutive")
+    pc = cols + 2 * (WINDOW[1] // 2)
+    t0 = dt.datetime(2023, 1, 12, 14, 25, 9)
+    starts = [t0 + dt.timedelta(days=12 * k) for k in range(n)]
+    with tcoh.Backend(s, pc, platform=0, device=0) as be:
+        coh, count, pairs = be.run_region(i, q, strip_rows=17)
+        est, mean, names = api.run_raw_region(
+            be, tmp_path / "stack_i.raw", tmp_path / "stack_q.raw", rows, cols,
+            tmp_path, "coh", shp_count="shp", pairs_prefix="p", starts=starts,
+            strip_rows=17)
+    assert names[0] == "p_20230112_20230124" and len(names) == n - 1
+    nan = np.isnan(coh)
+    f = np.fromfile(tmp_path / "coh.raw", np.uint32)
+    assert np.all(f[nan] == 0xFFFFFFFF) and np.array_equal(f[~nan], coh.view(np.uint32)[~nan])
+    c = np.fromfile(tmp_path / "shp.raw", np.int32)
+    assert np.array_equal(c, np.where(count < 0, np.iinfo(np.int32).min, count))
+    for p, name in enumerate(names):
+        g = np.fromfile(tmp_path / (name + ".raw"), np.float32)
+        assert np.array_equal(np.isnan(g), np.isnan(pairs[p]))
+        assert np.array_equal(g[~np.isnan(g)], pairs[p][~np.isnan(pairs[p])])
+    assert est == int(np.sum(~nan)) and est > 100
+    assert mean == pytest.approx(float(np.mean(coh[~nan], dtype=np.float64)), rel=1e-12)
