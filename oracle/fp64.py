@@ -26,6 +26,15 @@ computed in double precision from the float32 samples:
 
 Eigenvectors come from numpy.linalg.eigh (LAPACK), the same library the
 reference's own oracle uses (reference.py:148-157).
+
+With form="f16" or "bf16", the covariance is the one the library's mixed
+forms define (no reference counterpart): each sample split as hi + lo
+16-bit values (round to nearest even; FP16 after a power-of-two scale that
+puts the largest magnitude in [2^13, 2^14), BF16 unscaled) and every
+product taken as hi.hi + hi.lo + lo.hi, so C_form = C(hi + lo) - lo lo^H.
+This is the arithmetic of the form, not of the reference: its distance
+from the exact C is the form's own error, which an ill-conditioned EMI
+estimate amplifies like any other input error.
 """
 
 import numpy as np
@@ -48,15 +57,47 @@ def mask_bits(mask, nbits):
     return b[:, :nbits].astype(bool)
 
 
-def covariance(slc, accepted, origin, offsets):
+def covariance(slc, accepted, origin, offsets, lo=None):
     """Hermitian C = sum_k s_k s_k^H over accepted candidates, complex128.
 
     slc: complex (npad, N); accepted: bool (P, K); origin: window origin
-    pixel index (P,); offsets (K,).
+    pixel index (P,); offsets (K,). lo (complex (npad, N), optional): the
+    lo parts of a split form, whose lo lo^H products the form leaves out.
     """
     idx = origin[:, None] + offsets[None, :]
     x = slc[idx] * accepted[..., None]  # (P, K, N)
-    return np.einsum("pki,pkj->pij", x, x.conj(), optimize=True)
+    c = np.einsum("pki,pkj->pij", x, x.conj(), optimize=True)
+    if lo is not None:
+        y = lo[idx] * accepted[..., None]
+        c = c - np.einsum("pki,pkj->pij", y, y.conj(), optimize=True)
+    return c
+
+
+def round_bf16(x):
+    """float32 -> nearest BF16 (ties to even), as float32 (finite x)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def split_parts(slc32, form):
+    """(hi + lo, lo) of float32 samples under a mixed form's split, as
+    float64 arrays of the unscaled values."""
+    x = np.asarray(slc32, np.float32)
+    if form == "bf16":
+        scale = 1.0
+        rnd = round_bf16
+    else:
+        fin = np.abs(x[np.isfinite(x)])
+        mx = float(fin.max()) if fin.size else 0.0
+        scale = 2.0 ** (14 - np.frexp(np.float32(mx))[1]) if mx > 0 else 1.0
+        def rnd(v):
+            return np.asarray(v, np.float32).astype(np.float16).astype(np.float32)
+    xs = (x * np.float32(scale)).astype(np.float32)
+    hi = rnd(xs)
+    lo = rnd((xs - hi).astype(np.float32))
+    return ((hi.astype(np.float64) + lo.astype(np.float64)) / scale,
+            lo.astype(np.float64) / scale)
 
 
 def coherence(c, looks, bias):
@@ -193,7 +234,7 @@ def estimate(t, emi, loading=0.0):
 
 
 def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
-        chunk=4096, emi_loading=0.0, band=None):
+        chunk=4096, emi_loading=0.0, band=None, form=None):
     """FP64 coherence and pair coherences of the strip's output pixels.
 
     slc: float32 (npad, N, 2) or complex; count: int (rows*cols,) from the
@@ -208,7 +249,12 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
     """
     n = settings["ndates"]
     wa, wr = settings["win_az"], settings["win_rg"]
-    if slc.dtype != np.complex128:
+    lo_parts = None
+    if form in ("f16", "bf16"):
+        full, low = split_parts(np.asarray(slc, np.float32).reshape(-1, n, 2), form)
+        slc = full[..., 0] + 1j * full[..., 1]
+        lo_parts = low[..., 0] + 1j * low[..., 1]
+    elif slc.dtype != np.complex128:
         slc = np.asarray(slc, np.float32).reshape(-1, n, 2)
         slc = slc[..., 0].astype(np.float64) + 1j * slc[..., 1].astype(np.float64)
     if pixels is None:
@@ -235,7 +281,7 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
         y, x = p // cols, p % cols
         origin = y * padded_cols + x
         acc = mask_bits(mask[p], wa * wr)
-        c = covariance(slc, acc, origin, offs)
+        c = covariance(slc, acc, origin, offs, lo_parts)
         t = coherence(c, cnt[est], settings["bias"])
         out = np.flatnonzero(est) + lo
         if band is not None and settings["bias"]:

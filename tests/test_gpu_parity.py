@@ -64,6 +64,13 @@ def check_against_oracles(tcoh, cref, fp64, stack, opts, precision, tol=TOLERANC
     both = np.isfinite(coh) & np.isfinite(c64)
     ref_err = np.nan_to_num(np.abs(ocoh - c64), nan=0.0)
     allow = tol + 2.0 * near / (0.5 * n * (n - 1)) + ref_err
+    if s.emi and precision != "plain":
+        # A split form's products leave out lo.lo (2^-18 of a product for
+        # BF16, 2^-22 for FP16): the form's own distance from the exact
+        # estimate, which |T|^-1 amplifies like the FP32 rounding above.
+        cf, _, _, _ = fp64.run(slc, ocount, mask, cref.settings_dict(so), pcols,
+                               rows, cols, band=band, form=precision)
+        allow = allow + np.nan_to_num(np.abs(cf - c64), nan=0.0)
     err = np.abs(coh - c64)
     worst = np.argmax(np.where(both, err - allow, -np.inf)) if both.any() else 0
     assert not both.any() or err[worst] <= allow[worst], \
@@ -85,6 +92,36 @@ def test_reference_cases(tcoh, cref, fp64, case, precision):
     stack = tcoh.synth.ds_stack(case["n"], case["g"], seed=case["seed"])
     check_against_oracles(tcoh, cref, fp64, stack, case["opts"], precision,
                           tol_for(precision))
+
+
+# Shape edges: the group-size boundaries of the linking stage (N = 3, 4, 16
+# with 8 lanes a pixel; 17, 31, 32 with 16), a 41x11 window (15 mask
+# words) and a 33x31 window next to the 1024-pixel limit, across the SHP
+# tests, both estimators, bias correction and both pair modes.
+EDGE_CASES = [
+    dict(n=3, g=0.8, seed=11, opts=dict(window=(5, 3), min_shp=3)),
+    dict(n=4, g=0.6, seed=12, opts=dict(window=(3, 3), shp_test="tlog", estimator="emi",
+                                        min_shp=4, bias=True)),
+    dict(n=16, g=0.7, seed=13, opts=dict(window=(9, 5), shp_test="ad", min_shp=16,
+                                         pairs="all")),
+    dict(n=17, g=0.7, seed=14, opts=dict(window=(9, 5), estimator="emi", min_shp=17,
+                                         bias=True)),
+    dict(n=31, g=0.8, seed=15, opts=dict(shp_test="tlog", bias=True, pairs="consecutive")),
+    dict(n=32, g=0.85, seed=16, opts=dict(estimator="emi")),
+    dict(n=12, g=0.7, seed=17, opts=dict(window=(41, 11), bias=True)),
+    dict(n=5, g=0.7, seed=18, opts=dict(window=(33, 31), shp_test="ad", pairs="all")),
+]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("case", EDGE_CASES,
+                         ids=["n%d_%dx%d" % (c["n"], *c["opts"].get("window", (21, 7)))
+                              for c in EDGE_CASES])
+def test_shape_edges(tcoh, cref, fp64, case, precision):
+    stack = tcoh.synth.ds_stack(case["n"], case["g"], seed=case["seed"])
+    coh, count, _ = check_against_oracles(tcoh, cref, fp64, stack, case["opts"],
+                                          precision, tol_for(precision))
+    assert np.isfinite(coh).sum() > 200
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)

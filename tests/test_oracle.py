@@ -346,3 +346,34 @@ def test_emi_fallback_scene_reaches_the_jacobi_path(cref, fp64, tcoh):
     with np.errstate(all="ignore"):
         _piv, ok = fp64.cholesky_pivots(np.abs(t))
     assert 0.05 < np.mean(~ok) < 0.6, np.mean(~ok)
+
+
+def test_fp64_split_forms(fp64):
+    """The mixed forms' covariance in the FP64 oracle: round to nearest even
+    (BF16 ties to even by bits, FP16 by numpy's IEEE conversion), hi + lo
+    within the form's precision of the sample, and C_form = C(hi + lo) -
+    lo lo^H equal to the sum of the hi.hi, hi.lo and lo.hi products of the
+    real Gram matrices (C_re = Re Re^T + Im Im^T, C_im = Im Re^T - Re Im^T)."""
+    # 1 + 2^-8 is halfway between BF16 neighbours 1 and 1 + 2^-7: to even
+    x = np.array([1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -2.5, 0.0], np.float32)
+    assert list(fp64.round_bf16(x)) == [1.0, 1 + 2.0 ** -6, -2.5, 0.0]
+    rng = np.random.default_rng(3)
+    n, k = 7, 40
+    xs = (rng.normal(size=(k, n, 2)) * 300).astype(np.float32)
+    for form, prec in (("f16", 2.0 ** -21), ("bf16", 2.0 ** -15)):
+        full, lo = fp64.split_parts(xs, form)
+        assert np.all(np.abs(full - xs) <= prec * np.abs(xs) + 1e-30)
+        hi = full - lo
+        s_full = full[..., 0] + 1j * full[..., 1]
+        s_lo = lo[..., 0] + 1j * lo[..., 1]
+        acc = np.ones((1, k), bool)
+        c = fp64.covariance(s_full, acc, np.array([0]), np.arange(k), s_lo)[0]
+        rh, ih, rl, il = hi[..., 0], hi[..., 1], lo[..., 0], lo[..., 1]
+
+        def gram(a_h, a_l, b_h, b_l):
+            return a_h.T @ b_h + a_h.T @ b_l + a_l.T @ b_h
+
+        cre = gram(rh, rl, rh, rl) + gram(ih, il, ih, il)
+        cim = gram(ih, il, rh, rl) - gram(rh, rl, ih, il)
+        assert np.allclose(c.real, cre, rtol=1e-13, atol=1e-9)
+        assert np.allclose(c.imag, cim, rtol=1e-13, atol=1e-9)
