@@ -109,28 +109,49 @@ template <int N> __device__ __forceinline__ int lpos(int i, int j)
 #define TCOH_FMAC_BC(D, S, B, L)                                               \
     "v_fmac_f32_dpp " D ", " S ", " B " row_newbcast:" L                        \
     " row_mask:0xf bank_mask:0xf\n"
+/* the same with the lane's own operand negated by the DPP src1 modifier */
+#define TCOH_FMAC_BC_NEG(D, S, B, L)                                           \
+    "v_fmac_f32_dpp " D ", " S ", -" B " row_newbcast:" L                       \
+    " row_mask:0xf bank_mask:0xf\n"
+#define TCOH_MUL_BC(D, S, B, L)                                                \
+    "v_mul_f32_dpp " D ", " S ", " B " row_newbcast:" L                         \
+    " row_mask:0xf bank_mask:0xf\n"
 
 template <int L> struct RowOps {
-    /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c) for one column slot:
-       (ax, ay) += vx*nwx + vy*nwy + wx*nvx + wy*nvy,
-                   vy*nwx + vx*wcy + wy*nvx + wx*vcy */
+    /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c) for one column slot, with
+       (wcx, wcy, vcx, vcy) the lane's own w_c, v_c:
+       ax += -(vx*wcx + vy*wcy + wx*vcx + wy*vcy),
+       ay += -vy*wcx + vx*wcy - wy*vcx + wx*vcy */
     static __device__ __forceinline__ void update1(float &ax, float &ay, float vx,
                                                    float vy, float wx, float wy,
-                                                   float nwx, float nwy, float nvx,
-                                                   float nvy, float wcy, float vcy)
+                                                   float wcx, float wcy, float vcx,
+                                                   float vcy)
     {
         asm volatile("s_nop 1\n"
-                     TCOH_FMAC_BC("%0", "%2", "%6", "%12")
-                     TCOH_FMAC_BC("%0", "%3", "%7", "%12")
-                     TCOH_FMAC_BC("%0", "%4", "%8", "%12")
-                     TCOH_FMAC_BC("%0", "%5", "%9", "%12")
-                     TCOH_FMAC_BC("%1", "%3", "%6", "%12")
-                     TCOH_FMAC_BC("%1", "%2", "%10", "%12")
-                     TCOH_FMAC_BC("%1", "%5", "%8", "%12")
-                     TCOH_FMAC_BC("%1", "%4", "%11", "%12")
+                     TCOH_FMAC_BC_NEG("%0", "%2", "%6", "%10")
+                     TCOH_FMAC_BC_NEG("%0", "%3", "%7", "%10")
+                     TCOH_FMAC_BC_NEG("%0", "%4", "%8", "%10")
+                     TCOH_FMAC_BC_NEG("%0", "%5", "%9", "%10")
+                     TCOH_FMAC_BC_NEG("%1", "%3", "%6", "%10")
+                     TCOH_FMAC_BC("%1", "%2", "%7", "%10")
+                     TCOH_FMAC_BC_NEG("%1", "%5", "%8", "%10")
+                     TCOH_FMAC_BC("%1", "%4", "%9", "%10")
                      : "+v"(ax), "+v"(ay)
-                     : "v"(vx), "v"(vy), "v"(wx), "v"(wy), "v"(nwx), "v"(nwy),
-                       "v"(nvx), "v"(nvy), "v"(wcy), "v"(vcy), "n"(L));
+                     : "v"(vx), "v"(vy), "v"(wx), "v"(wy), "v"(wcx), "v"(wcy),
+                       "v"(vcx), "v"(vcy), "n"(L));
+    }
+    /* The first row of the partial sums: written, not accumulated. */
+    static __device__ __forceinline__ void matvec1_first(float &sr, float &sia,
+                                                         float &sib, float vx,
+                                                         float vy, float ax, float ay)
+    {
+        asm volatile("s_nop 1\n"
+                     TCOH_MUL_BC("%0", "%3", "%5", "%7")
+                     TCOH_FMAC_BC("%0", "%4", "%6", "%7")
+                     TCOH_MUL_BC("%1", "%4", "%5", "%7")
+                     TCOH_MUL_BC("%2", "%3", "%6", "%7")
+                     : "=&v"(sr), "=&v"(sia), "=&v"(sib)
+                     : "v"(vx), "v"(vy), "v"(ax), "v"(ay), "n"(L));
     }
     /* w partial sums: sr += vx*ax + vy*ay, sia += vy*ax, sib += vx*ay */
     static __device__ __forceinline__ void matvec1(float &sr, float &sia, float &sib,
@@ -186,24 +207,25 @@ __device__ __forceinline__ void update_row64(int r, int k, float4 t, float2 (&a)
     F(14) F(15)
 
 __device__ __forceinline__ void row_update1(int l, float &ax, float &ay, float2 v,
-                                            float2 w, float2 nw, float2 nv,
-                                            float wcy, float vcy)
+                                            float2 w, float2 wc, float2 vc)
 {
 #define TCOH_CASE(L)                                                           \
     case L:                                                                    \
-        RowOps<L>::update1(ax, ay, v.x, v.y, w.x, w.y, nw.x, nw.y, nv.x, nv.y, \
-                           wcy, vcy);                                          \
+        RowOps<L>::update1(ax, ay, v.x, v.y, w.x, w.y, wc.x, wc.y, vc.x, vc.y); \
         break;
     switch (l) { TCOH_ROW_CASES(TCOH_CASE) }
 #undef TCOH_CASE
 }
 
-__device__ __forceinline__ void row_matvec1(int l, float &sr, float &sia, float &sib,
-                                            float2 v, float2 a)
+__device__ __forceinline__ void row_matvec1(int l, bool first, float &sr, float &sia,
+                                            float &sib, float2 v, float2 a)
 {
 #define TCOH_CASE(L)                                                           \
     case L:                                                                    \
-        RowOps<L>::matvec1(sr, sia, sib, v.x, v.y, a.x, a.y);                  \
+        if (first)                                                             \
+            RowOps<L>::matvec1_first(sr, sia, sib, v.x, v.y, a.x, a.y);        \
+        else                                                                   \
+            RowOps<L>::matvec1(sr, sia, sib, v.x, v.y, a.x, a.y);              \
         break;
     switch (l) { TCOH_ROW_CASES(TCOH_CASE) }
 #undef TCOH_CASE
@@ -352,7 +374,8 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                 for (int m = 0; m < CPL; m++) {
                     if (min(m * G + G - 1, N - 1) < k + 1)
                         continue;
-                    row_matvec1(j % G, sr2[m], sia[m], sib[m], vc[j / G], A[m][j]);
+                    row_matvec1(j % G, j == k + 1, sr2[m], sia[m], sib[m], vc[j / G],
+                                A[m][j]);
                 }
                 continue;
             }
@@ -451,7 +474,7 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                     if (min(m * G + G - 1, N - 1) < k + 1)
                         continue;
                     row_update1(r % G, A[m][r].x, A[m][r].y, vc[r / G], wc[r / G],
-                                nw[m], nv[m], wc[m].y, vc[m].y);
+                                wc[m], vc[m]);
                 }
                 continue;
             }
