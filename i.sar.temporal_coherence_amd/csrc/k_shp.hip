@@ -221,28 +221,45 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                     for (int i = 0; i < NB; i++)
                         if (i < n)
                             bcol[i][threadIdx.x] = b[i];
-                    /* the current sample of each side, reloaded only for
-                       the side that advances */
+                    /* the current sample of each side; the side that
+                       advances is reloaded by one LDS read from the
+                       selected column (no branch) */
                     float va = acol[0][threadIdx.x], vb = bcol[0][threadIdx.x];
 
-                    for (int j = 1; j < 2 * n; j++) {
-                        bool take_a;
+                    /* One step of the merge; returns the step's table
+                       term, whose load is left in flight. */
+                    auto step = [&](int j) -> float {
+                        const bool take_a = ia < n && (ib >= n || va <= vb);
 
-                        if (ia >= n)
-                            take_a = false;
-                        else if (ib >= n)
-                            take_a = true;
-                        else
-                            take_a = va <= vb;
                         m += take_a;
                         ia += take_a;
                         ib += !take_a;
-                        if (take_a)
-                            va = acol[min(ia, n - 1)][threadIdx.x];
-                        else
-                            vb = bcol[min(ib, n - 1)][threadIdx.x];
-                        s = s + adtab[j * (n + 1) + m];
+                        const float *src = take_a ? &acol[min(ia, n - 1)][threadIdx.x]
+                                                  : &bcol[min(ib, n - 1)][threadIdx.x];
+                        const float v = *src;
+
+                        va = take_a ? v : va;
+                        vb = take_a ? vb : v;
+                        return adtab[j * (n + 1) + m];
+                    };
+                    /* Eight steps issue their table loads before the first
+                       of their terms is added, so the loads overlap the
+                       walk's LDS chain; the terms are still added one by
+                       one in j order. */
+                    int j = 1;
+
+                    for (; j + 7 < 2 * n; j += 8) {
+                        float t[8];
+
+#pragma unroll
+                        for (int u = 0; u < 8; u++)
+                            t[u] = step(j + u);
+#pragma unroll
+                        for (int u = 0; u < 8; u++)
+                            s = s + t[u];
                     }
+                    for (; j < 2 * n; j++)
+                        s = s + step(j);
                     /* plain operators: contraction is off in this file
                        (hipcc's __f*_rn helpers are contractable operators
                        and a native square root) */
