@@ -357,13 +357,15 @@ def main():
     avg_launch_s = st_ms[dom] * 1e-3 / max(1.0, nl[dom])
     achieved = flop / max(1.0, nl[dom]) / avg_launch_s
     peak = flop / t_bound
+    # the HBM traffic was measured (rocprofv3 counters) on this workload only
+    traffic_measured = (args.config == 2 and args.precision == "f16"
+                        and rows == 2048 and cols == 2048)
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": TRAFFIC_F16_N30 if (args.config == 2 and args.precision == "f16"
-                                               and rows == 2048 and cols == 2048) else None,
-                "traffic_source": "profiles/k_link_n30_2048x2048_kernel_stats.md "
-                                  "(rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
-                                  "calibrated on a 1 GiB copy)",
+                "traffic": TRAFFIC_F16_N30 if traffic_measured else None,
+                "traffic_source": ("profiles/k_link_n30_2048x2048_kernel_stats.md "
+                                   "(rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
+                                   "calibrated on a 1 GiB copy)") if traffic_measured else None,
                 "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,
                 "launches_per_step": float(nl[dom])}
 

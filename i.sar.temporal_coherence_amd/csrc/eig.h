@@ -509,13 +509,15 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
 /* Eigenvalue of rank idx (0 smallest) of the tridiagonal in LDS by G-way
    multisection of Sturm counts on the Gershgorin interval, with the
    reference's pivot guard (tcoh.cl:295-338). Returns lambda and norm.
-   TOP (idx = N - 1, the largest): a point is above lambda exactly when
-   every pivot is negative, so the count is a flag; and the search stops
-   once the interval is within 2^-20 of |lambda| (a few ulp; inverse
-   iteration from such a shift converges as from the exact one), two rounds
-   of 17x narrowing short of the float resolution at which the plain rule
-   (no further progress) stops. */
-template <int N, int G, bool TOP>
+   EXT (idx = N - 1 or 0, an extreme eigenvalue): a point is above lambda
+   exactly when every pivot is negative (largest) or any is (smallest), so
+   the count is a flag; and the search stops once the interval is within
+   2^-20 of |lambda| (a few ulp; inverse iteration from such a shift
+   converges as from the exact one), two rounds of 17x narrowing short of
+   the float resolution at which the plain rule (no further progress)
+   stops. Near lambda = 0 the relative test never holds and the plain rule
+   ends the search. */
+template <int N, int G, bool EXT>
 __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
                                                     int idx, int lg,
                                                     float *norm)
@@ -534,13 +536,14 @@ __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
     const float pivmin = 1.0e-30f + 1.0e-7f * *norm * 1.0e-7f;
 
     for (int it = 0; it < 24; it++) {
-        if (TOP && hi - lo <= 0x1p-20f * fmaxf(fabsf(lo), fabsf(hi)))
+        if (EXT && hi - lo <= 0x1p-20f * fmaxf(fabsf(lo), fabsf(hi)))
             break;
         const float step = (hi - lo) * (1.0f / (G + 1));
         const float x = fminf(lo + step * (float)(lg + 1), hi);
         float qv = sh.d[0] - x;
         int cnt = qv < 0.0f;
-        bool allneg = qv < 0.0f;
+        bool allneg = qv < 0.0f, anyneg = qv < 0.0f;
+        const bool top = idx != 0;
 
         /* The count only decides a side of x: the hardware reciprocal
            (1 ulp) is as good as a rounded division here. */
@@ -549,12 +552,14 @@ __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
             if (fabsf(qv) < pivmin)
                 qv = -pivmin;
             qv = sh.d[i] - x - sh.e2[i - 1] * __builtin_amdgcn_rcpf(qv);
-            if (TOP)
+            if (EXT && top)
                 allneg &= qv < 0.0f;
+            else if (EXT)
+                anyneg |= qv < 0.0f;
             else
                 cnt += qv < 0.0f;
         }
-        const bool above = TOP ? allneg : cnt > idx;
+        const bool above = EXT ? (top ? allneg : anyneg) : cnt > idx;
         const float nlo = gmax<G>(above ? lo : x);
         const float nhi = gmin<G>(above ? x : hi);
 
@@ -720,9 +725,9 @@ __device__ __forceinline__ void hermitian_eigenvector(float2 (&A)[CPL][N],
     float norm;
 
     tridiagonalize<N, G, CPL, EXACT>(A, sh, lg);
-    /* EVD (the largest, well away from 0) stops on the relative width;
-       EMI's smallest eigenvalue runs the reference's rule to the end */
-    const float lambda = tridiag_eigenvalue<N, G, !EXACT>(sh, idx, lg, &norm);
+    /* both estimators take an extreme eigenvalue (EVD the largest, EMI the
+       smallest) */
+    const float lambda = tridiag_eigenvalue<N, G, true>(sh, idx, lg, &norm);
     tridiag_eigenvector<N, EXACT>(sh, lambda, norm, lg);
     back_transform<N, G, CPL>(A, sh, lg, z);
 }
