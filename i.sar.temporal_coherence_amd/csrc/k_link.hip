@@ -23,6 +23,12 @@ template <int N> struct LinkShape {
     static constexpr int TL_BYTES = (PPW * NT * 8 + 15) & ~15;
     static constexpr int EIG_BYTES = PPW * (int)sizeof(EigLds<N>);
     static constexpr int EMI_BYTES = PPW * (int)sizeof(EmiLds<N>);
+    /* A group that leaves its Jacobi pixels to the queue uses only the
+       Cholesky part of EmiLds: smaller strides let one more group share a
+       compute unit's LDS (at N=30, 35 KB a group instead of 51 KB: 4 groups
+       per CU, one per SIMD, instead of 3). */
+    static constexpr int CH_STRIDE = ((int)sizeof(EmiLds<N>::ch) + 15) & ~15;
+    static constexpr int CH_BYTES = PPW * CH_STRIDE;
 };
 
 template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
@@ -62,8 +68,14 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     float2 *tl = reinterpret_cast<float2 *>(lds);
     unsigned char *scratch = lds + S::TL_BYTES;
-    EmiLds<N> *ems = reinterpret_cast<EmiLds<N> *>(scratch + scratch_bytes);
+    /* With a queue, a group whose |T| needs the Jacobi pseudo-inverse
+       leaves it to k_emi_queue, where every group of a wavefront needs it:
+       here one such group would hold up the wavefront's other three. */
+    const bool defer = EMI && q.t != nullptr;
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
+    EmiLds<N> *em = reinterpret_cast<EmiLds<N> *>(
+        scratch + scratch_bytes +
+        (EMI ? gi * (defer ? S::CH_STRIDE : (int)sizeof(EmiLds<N>)) : 0));
     int r, tx;
 
     if (!xcd_tile(blockIdx.x, nrows, (kp.cols + PPW - 1) / PPW, &r, &tx))
@@ -86,13 +98,9 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
                                      scratch, tl, pairs);
     wave_sync();
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(scratch);
-    /* With a queue, a group whose |T| needs the Jacobi pseudo-inverse
-       leaves it to k_emi_queue, where every group of a wavefront needs it:
-       here one such group would hold up the wavefront's other three. */
-    const bool defer = EMI && q.t != nullptr;
     bool deferred;
-    const float gam = link_group<N, G, EMI>(kp, tl + gi * NT, est, lg, sh[gi],
-                                            ems[EMI ? gi : 0], !defer, deferred);
+    const float gam = link_group<N, G, EMI>(kp, tl + gi * NT, est, lg, sh[gi], *em,
+                                            !defer, deferred);
 
     if (defer) {
         const unsigned long long b = __ballot(lg == 0 && deferred);
@@ -183,7 +191,8 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
     typedef LinkShape<N> S;
     const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW));
     const int scratch = link_scratch_bytes<N>(kp, precision);
-    const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::EMI_BYTES : 0);
+    const size_t lds = (size_t)S::TL_BYTES + scratch +
+                       (kp.emi ? (qt ? S::CH_BYTES : S::EMI_BYTES) : 0);
 
 #define TCOH_LINK_LAUNCH(EMI)                                                   \
     do {                                                                        \
