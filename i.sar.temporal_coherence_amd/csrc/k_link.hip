@@ -127,7 +127,8 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
         coh[p] = gam;
 }
 
-/* The deferred EMI pixels, PPW to a wavefront: T from the queue, the
+/* The deferred EMI pixels, PPW to a wavefront: T read from the queue in
+   place (no LDS copy, so that four groups share a compute unit's LDS), the
    Jacobi pseudo-inverse of |T| (the Cholesky test fails again, as it did in
    k_link), then the linking as in k_link. */
 template <int N>
@@ -140,20 +141,16 @@ __global__ __launch_bounds__(64, 1) void k_emi_queue(KParams kp,
     typedef LinkShape<N> S;
     constexpr int G = S::G, PPW = S::PPW, NT = S::NT;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    float2 *tl = reinterpret_cast<float2 *>(lds);
-    EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(lds + S::TL_BYTES);
-    EmiLds<N> *ems = reinterpret_cast<EmiLds<N> *>(lds + S::TL_BYTES + S::EIG_BYTES);
+    EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(lds);
+    EmiLds<N> *ems = reinterpret_cast<EmiLds<N> *>(lds + S::EIG_BYTES);
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
     const int slot = blockIdx.x * PPW + gi;
     const bool live = slot < qcount;
-
-    if (live)
-        for (int t = lg; t < NT; t += G)
-            tl[gi * NT + t] = qt[(size_t)slot * NT + t];
-    wave_sync();
+    /* a group past the end reads nothing (not estimated) */
+    const float2 *tq = qt + (size_t)(live ? slot : 0) * NT;
     bool deferred;
-    const float gam = link_group<N, G, 1>(kp, tl + gi * NT, live, lg, sh[gi], ems[gi],
-                                          true, deferred);
+    const float gam = link_group<N, G, 1>(kp, tq, live, lg, sh[gi], ems[gi], true,
+                                          deferred);
 
     if (live && lg == 0)
         coh[qpix[slot]] = gam;
@@ -164,7 +161,7 @@ hipError_t launch_emi_queue_n(const KParams &kp, const float2 *qt, const int *qp
                               int qcount, float *coh, hipStream_t s)
 {
     typedef LinkShape<N> S;
-    const size_t lds = (size_t)S::TL_BYTES + S::EIG_BYTES + S::EMI_BYTES;
+    const size_t lds = (size_t)S::EIG_BYTES + S::EMI_BYTES;
 
     if (qcount <= 0)
         return hipSuccess;
