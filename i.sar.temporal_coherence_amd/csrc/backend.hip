@@ -428,11 +428,22 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
             tm.mark(stage, st, start);
     };
 
+    const bool mixed = o->precision != TCOH_PLAIN;
+    const int bf16 = o->precision == TCOH_MIXED_BF16;
+    /* The FP16 split's scale comes from the strip's largest amplitude,
+       which the preparation kernel folds into this word: no separate pass
+       over the samples and no host round trip. */
+    unsigned int *absmax = nullptr;
+
+    if (mixed && !bf16) {
+        o->absmax.ensure(sizeof(unsigned int), "hipMalloc(absmax)");
+        absmax = o->absmax.as<unsigned int>();
+    }
     if (timing)
         tm.begin();
     mark(0, true);
     check(launch_prep(kp, (const float *)amp, o->ampT.as<float>(),
-                      (const unsigned char *)valid, o->lstat.as<float2>(), st),
+                      (const unsigned char *)valid, o->lstat.as<float2>(), absmax, st),
           "prep kernel");
     mark(0, false);
     mark(1, true);
@@ -442,42 +453,14 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
           "shp kernel");
     mark(1, false);
 
-    const bool mixed = o->precision != TCOH_PLAIN;
     if (mixed) {
         /* Samples as (hi, lo) 16-bit pairs, one zero pixel appended for
-           the padding of candidate lists. Power-of-two scale: T is
-           invariant to a common factor of the samples. */
-        const int bf16 = o->precision == TCOH_MIXED_BF16;
-        float scale = 1.0f;
-
+           the padding of candidate lists. */
         o->hl.ensure((npad + 1) * shp_bucket(n) * 2 * sizeof(uint32_t), "hipMalloc(hl)");
-        if (!bf16) {
-            unsigned int bits = 0;
-
-            o->absmax.ensure(sizeof(unsigned int), "hipMalloc(absmax)");
-            check(launch_absmax((const float2 *)slc, npad * n,
-                                o->absmax.as<unsigned int>(), st),
-                  "absmax kernel");
-            check(hipMemcpyAsync(&bits, o->absmax.p, sizeof(bits),
-                                 hipMemcpyDeviceToHost, st),
-                  "hipMemcpy(absmax)");
-            check(hipStreamSynchronize(st), "hipStreamSynchronize");
-            float mx;
-            memcpy(&mx, &bits, sizeof(mx));
-            if (mx > 0.0f && isfinite(mx)) {
-                int ex;
-
-                frexpf(mx, &ex);
-                /* largest magnitude in [2^13, 2^14): hi fits f16, products
-                   and sums of up to 1024 candidates stay far below FP32
-                   overflow */
-                scale = ldexpf(1.0f, 14 - ex);
-            }
-        }
         /* counted with the preparation stage */
         mark(0, true);
-        check(launch_split(kp, (const float2 *)slc, o->hl.as<uint32_t>(), bf16,
-                           scale, st),
+        check(launch_split(kp, (const float2 *)slc, o->hl.as<uint32_t>(), bf16, absmax,
+                           st),
               "split kernel");
         mark(0, false);
     }

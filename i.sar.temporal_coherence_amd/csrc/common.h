@@ -383,15 +383,13 @@ __host__ __device__ __forceinline__ int upper_index(int n, int i, int j)
    code; s is the handle's stream. */
 hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
                        const unsigned char *valid, float2 *lstat,
-                       hipStream_t s);
+                       unsigned int *absmax, hipStream_t s);
 hipError_t launch_shp(const KParams &kp, const float *ampT,
                       const unsigned char *valid, const float2 *lstat,
                       const float *adtab, uint32_t *mask, int *count,
                       hipStream_t s);
 hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
-                        int bf16, float scale, hipStream_t s);
-hipError_t launch_absmax(const float2 *slc, size_t n, unsigned int *out,
-                         hipStream_t s);
+                        int bf16, const unsigned int *absmax, hipStream_t s);
 /* Strip preparation (main.c:52-127) on the device. */
 struct StripArgs {
     const float *i, *q;       /* date planes [n][nrows][ncols], NaN null */

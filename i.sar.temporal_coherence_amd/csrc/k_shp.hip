@@ -39,7 +39,10 @@ __device__ __forceinline__ float logf_rounded(float x)
 }
 
 /* ampT[k][pix] = amp[pix][k]; lstat[pix] = (mean, var) of log amplitudes
-   of valid pixels when the t-test is used. A block stages its PREP_PX
+   of valid pixels when the t-test is used; with absmax, the largest finite
+   amplitude of the strip is folded into *absmax (bits of a non-negative
+   float, one atomic per wavefront) for the split forms' scale. A block
+   stages its PREP_PX
    pixels' amplitudes (contiguous in amp) in LDS with coalesced reads, then
    writes the date planes; a row of the tile is padded to n + 1 words so
    that the per-pixel reads do not collide in LDS banks. */
@@ -49,7 +52,8 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
                                                  const float *__restrict__ amp,
                                                  float *__restrict__ ampT,
                                                  const unsigned char *__restrict__ valid,
-                                                 float2 *__restrict__ lstat)
+                                                 float2 *__restrict__ lstat,
+                                                 unsigned int *__restrict__ absmax)
 {
     __shared__ float tile[PREP_PX * (64 + 1)]; /* up to 64 dates */
     const int n = kp.n, n1 = n + 1;
@@ -60,10 +64,25 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
 
     /* t < 128 * 64: (t + 0.5) / n is at least 0.5 / n away from an
        integer, far more than the float product's error */
+    unsigned int mbits = 0u;
+
     for (int t = threadIdx.x; t < np * n; t += PREP_PX) {
         const int pp = (int)(((float)t + 0.5f) * inv_n);
+        const float v = src[t];
 
-        tile[pp * n1 + (t - pp * n)] = src[t];
+        tile[pp * n1 + (t - pp * n)] = v;
+        if (isfinite(v))
+            mbits = max(mbits, __float_as_uint(fabsf(v)));
+    }
+    if (absmax) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1)
+            mbits = max(mbits, (unsigned int)__shfl_xor((int)mbits, o, 64));
+        /* read first: the stored maximum rises fast, so most wavefronts
+           skip the atomic and its contention on the one address */
+        if ((threadIdx.x & 63) == 0 &&
+            mbits > __hip_atomic_load(absmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(absmax, mbits);
     }
     __syncthreads();
     if ((int)threadIdx.x >= np)
@@ -288,12 +307,16 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
     count[out] = looks;
 }
 
+__global__ void k_zero_u32(unsigned int *out) { *out = 0u; }
+
 hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
                        const unsigned char *valid, float2 *lstat,
-                       hipStream_t s)
+                       unsigned int *absmax, hipStream_t s)
 {
+    if (absmax)
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, absmax);
     hipLaunchKernelGGL(k_prep, dim3((kp.npad + PREP_PX - 1) / PREP_PX), dim3(PREP_PX),
-                       0, s, kp, amp, ampT, valid, lstat);
+                       0, s, kp, amp, ampT, valid, lstat, absmax);
     return hipGetLastError();
 }
 

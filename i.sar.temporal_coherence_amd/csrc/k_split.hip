@@ -42,7 +42,7 @@ __device__ __forceinline__ float bf16_value(uint32_t u)
 __global__ __launch_bounds__(256) void k_split(KParams kp,
                                               const float2 *__restrict__ slc,
                                               uint2 *__restrict__ hl, int bf16,
-                                              float scale)
+                                              const unsigned int *__restrict__ absmax)
 {
     const int nb = shp_bucket(kp.n);
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -50,6 +50,22 @@ __global__ __launch_bounds__(256) void k_split(KParams kp,
 
     if (i >= total)
         return;
+    /* FP16: the power of two that puts the strip's largest amplitude in
+       [2^13, 2^14), so every component's hi part fits FP16 and products
+       and sums of up to 1024 candidates stay far below FP32 overflow; T is
+       invariant to the common factor. BF16 keeps the FP32 range. */
+    float scale = 1.0f;
+
+    if (absmax) {
+        const float mx = __uint_as_float(*absmax);
+
+        if (mx > 0.0f && isfinite(mx)) {
+            int ex;
+
+            frexpf(mx, &ex);
+            scale = ldexpf(1.0f, 14 - ex);
+        }
+    }
     const size_t pix = i / nb;
     const int d = (int)(i - pix * nb);
     const float2 v = (pix < (size_t)kp.npad && d < kp.n) ? slc[pix * kp.n + d]
@@ -74,45 +90,13 @@ __global__ __launch_bounds__(256) void k_split(KParams kp,
     hl[i] = make_uint2(w[0], w[1]);
 }
 
-/* Largest |re|, |im| of the strip as float bits (non-negative floats
-   order like their bit patterns). */
-__global__ __launch_bounds__(256) void k_absmax(const float2 *__restrict__ slc,
-                                               size_t n,
-                                               unsigned int *__restrict__ out)
-{
-    unsigned int m = 0;
-
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * 256) {
-        const float2 v = slc[i];
-        const float a = fmaxf(fabsf(v.x), fabsf(v.y));
-
-        if (isfinite(a))
-            m = max(m, __float_as_uint(a));
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(out, m);
-}
-
-__global__ void k_zero_u32(unsigned int *out) { *out = 0u; }
-
 hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
-                        int bf16, float scale, hipStream_t s)
+                        int bf16, const unsigned int *absmax, hipStream_t s)
 {
     const size_t total = ((size_t)kp.npad + 1) * shp_bucket(kp.n);
 
     hipLaunchKernelGGL(k_split, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                       0, s, kp, slc, (uint2 *)hl, bf16, scale);
+                       0, s, kp, slc, (uint2 *)hl, bf16, absmax);
     return hipGetLastError();
 }
 
-hipError_t launch_absmax(const float2 *slc, size_t n, unsigned int *out,
-                         hipStream_t s)
-{
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, out);
-    hipLaunchKernelGGL(k_absmax, dim3(1024), dim3(256), 0, s, slc, n, out);
-    return hipGetLastError();
-}

@@ -30,7 +30,7 @@ reference's own oracle uses (reference.py:148-157).
 With form="f16" or "bf16", the covariance is the one the library's mixed
 forms define (no reference counterpart): each sample split as hi + lo
 16-bit values (round to nearest even; FP16 after a power-of-two scale that
-puts the largest magnitude in [2^13, 2^14), BF16 unscaled) and every
+puts the largest amplitude in [2^13, 2^14), BF16 unscaled) and every
 product taken as hi.hi + hi.lo + lo.hi, so C_form = C(hi + lo) - lo lo^H.
 This is the arithmetic of the form, not of the reference: its distance
 from the exact C is the form's own error, which an ill-conditioned EMI
@@ -88,7 +88,8 @@ def split_parts(slc32, form):
         scale = 1.0
         rnd = round_bf16
     else:
-        fin = np.abs(x[np.isfinite(x)])
+        amp = np.sqrt(x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]).astype(np.float32)
+        fin = amp[np.isfinite(amp)]
         mx = float(fin.max()) if fin.size else 0.0
         scale = 2.0 ** (14 - np.frexp(np.float32(mx))[1]) if mx > 0 else 1.0
         def rnd(v):
