@@ -209,14 +209,20 @@ struct tcoh_raw_out *tcoh_raw_create(const char *dir, const char *output,
 int tcoh_raw_put_strip(struct tcoh_raw_out *w, int rows, const float *coh,
                        const int *count, const float *pairs)
 {
-    const size_t plane = (size_t)rows * w->cols;
     const uint32_t fnull = 0xffffffffu;
     const int32_t cnull = INT32_MIN;
-    uint32_t *fb = malloc((size_t)w->cols * sizeof(uint32_t));
-    int32_t *cb = malloc((size_t)w->cols * sizeof(int32_t));
-    int ok = fb && cb;
+    size_t plane;
+    uint32_t *fb;
+    int32_t *cb;
+    int ok;
 
     tcoh_clear_error();
+    if (!w || rows < 0)
+        return fail("Invalid map handle or row count for", "tcoh_raw_put_strip");
+    plane = (size_t)rows * w->cols;
+    fb = malloc((size_t)w->cols * sizeof(uint32_t));
+    cb = malloc((size_t)w->cols * sizeof(int32_t));
+    ok = fb && cb;
     if (!ok)
         fail("Out of memory for a row of", w->paths[0]);
     else if (w->written + rows > w->rows)
@@ -288,6 +294,8 @@ int tcoh_raw_close(struct tcoh_raw_out *w, long *estimated, double *mean)
     int rc = 0;
 
     tcoh_clear_error();
+    if (!w)
+        return fail("Invalid map handle for", "tcoh_raw_close");
     if (w->written != w->rows)
         rc = fail("Fewer rows written than the map holds:", w->paths[0]);
     if (!rc)

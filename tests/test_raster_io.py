@@ -129,3 +129,10 @@ def test_raw_map_reads_in_place(api, tmp_path):
     assert b"shorter than its declared size" in L.tcoh_last_error()
     assert L.tcoh_raw_map(str(tmp_path / "none.raw").encode(), 4, C.byref(addr)) == -1
     assert b"Unable to open" in L.tcoh_last_error()
+
+
+def test_raw_maps_reject_a_missing_handle(api):
+    L = api.lib()
+    assert L.tcoh_raw_put_strip(None, 1, None, None, None) == -1
+    assert b"Invalid map handle" in L.tcoh_last_error()
+    assert L.tcoh_raw_close(None, None, None) == -1
