@@ -124,6 +124,25 @@ def config5_inputs(tcoh, torch, s, n, rows, cols, hr):
     return i, q, heights, g
 
 
+def cpu_threads():
+    """The CPU baseline's threads: every core this process may use, at most
+    OMP_NUM_THREADS when the environment sets it (the GPU box's CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def flops_eigen(n):
     """Algorithmic FLOPs of one estimated pixel's eigen stage: Hermitian
     tridiagonal reduction (16/3 N^3) plus normalisation and fit 8N(N-1)."""
@@ -156,6 +175,36 @@ def run_form(tcoh, s, pcols, bufs, rows, cols, precision, steps, warmup, torch):
     return ms_step, stage_ms / steps, launches / steps
 
 
+def run_graph(tcoh, s, pcols, bufs, rows, cols, precision, steps, torch, want):
+    """The same step in graph mode (one captured HIP graph replayed per
+    step, tcoh_set_graph) on a stream of torch's: ms per step, and whether
+    the replayed maps equal the eager ones bit for bit."""
+    slc, amp, valid, coh, count, pairs = bufs
+    st = torch.cuda.Stream()
+    be = tcoh.Backend(s, pcols, platform=0, device=0, precision=precision)
+    be.set_graph(True)
+    args = (slc.data_ptr(), amp.data_ptr(), valid.data_ptr(), slc.shape[0], rows, cols,
+            coh.data_ptr(), count.data_ptr(), pairs.data_ptr())
+    torch.cuda.synchronize()
+    for _ in range(2):  # eager + capture, then one replay
+        be.run_device(*args, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        be.run_device(*args, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    be.close()
+    same = bool(np.array_equal(coh.cpu().numpy().view(np.uint32), want[0].view(np.uint32))
+                and np.array_equal(count.cpu().numpy(), want[1]))
+    return (t1 - t0) * 1e3 / steps, same
+
+
+# Bias-threshold band of each form's m2c rounding (as tests/test_gpu_parity.py
+# derives it): the oracle takes the form's own decision inside it.
+M2C_BAND = {"plain": 1e-6, "f16": 1e-6, "bf16": 1e-5, "emi": 1e-10}
+
+
 def pixel_conditioning(fp64, slc_c, count, mask, sd, pcols, size, k):
     """Why crop pixel k may differ: in double, its smallest |m2c| (bias
     threshold), the relative gap of T's two largest eigenvalues, and its
@@ -177,10 +226,39 @@ def pixel_conditioning(fp64, slc_c, count, mask, sd, pcols, size, k):
     return out
 
 
+def band_maps(tcoh, torch, s_opts, n, slc, amp, valid, pcols, y0, size, cols, precision):
+    """The crop's rows of the frame again, through tcoh_run_device_phases on
+    the frame's own device buffers (a full-width band of output rows y0 ..
+    y0 + size - 1) with all pairs: |T_ij| of every pair i < j and the
+    linked phases of that band. Each output pixel depends on its window
+    only (also in the split forms), so the band's gamma equals the frame
+    run's bit for bit, which the caller checks."""
+    wa = s_opts["window"][0]
+    s_all = tcoh.make_settings(n, **dict(s_opts, pairs="all"))
+    nt = n * (n - 1) // 2
+    coh = torch.empty((size, cols), dtype=torch.float32, device="cuda")
+    count = torch.empty((size, cols), dtype=torch.int32, device="cuda")
+    pairs = torch.empty((nt, size, cols), dtype=torch.float32, device="cuda")
+    ph = torch.empty((n, size, cols), dtype=torch.float32, device="cuda")
+    off = y0 * pcols
+    st = torch.cuda.current_stream()
+    with tcoh.Backend(s_all, pcols, platform=0, device=0, precision=precision) as be:
+        be.run_device(slc.data_ptr() + off * n * 8, amp.data_ptr() + off * n * 4,
+                      valid.data_ptr() + off, size + wa - 1, size, cols, coh.data_ptr(),
+                      count.data_ptr(), pairs.data_ptr(), stream=st.cuda_stream,
+                      phases_ptr=ph.data_ptr())
+        torch.cuda.synchronize()
+    return (coh.cpu().numpy(), count.cpu().numpy(), pairs.cpu().numpy(), ph.cpu().numpy())
+
+
 def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, n, s_opts, y0, x0, size,
-                  gpu_maps):
-    """Oracle SHP counts and FP64 gamma/pairs on a size x size crop; the
-    FP32 C oracle's own gamma error beside the GPU forms'."""
+                  gpu_maps, band_of):
+    """Oracle SHP counts and FP64 gamma, pairs and linked phases on a size x
+    size crop, against each GPU form: the FP64 oracle takes the form's own
+    zero / non-zero bias decision where |m2c| is within the form's rounding
+    of the threshold (band_of[form][0]: band, all-pairs magnitudes and
+    phases of the crop's rows); everything else is FP64. The FP32 C
+    oracle's own gamma error is reported beside the forms'."""
     wa, wr = s_opts["window"]
     pr_c, pc_c = size + wa - 1, size + wr - 1
     slc3 = slc.reshape(-1, pcols, n * 2)[y0:y0 + pr_c, x0:x0 + pc_c]
@@ -192,24 +270,42 @@ def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, n, s_opts, y0, x0, s
                                         nthreads=min(16, os.cpu_count() or 1))
     s32 = np.ascontiguousarray(slc3, np.float32).reshape(-1, n, 2)
     slc_c = s32[..., 0].astype(np.float64) + 1j * s32[..., 1].astype(np.float64)
-    c64, p64, near, _ = fp64.run(slc_c, ocount, mask, sd, pc_c, size, size, band=1e-5)
-    d32 = np.abs(ocoh - c64)
+    c64_plain = fp64.run(slc_c, ocount, mask, sd, pc_c, size, size)[0]
+    d32 = np.abs(ocoh - c64_plain)
     out = {"oracle_fp32": {"max_abs_dgamma": float(np.nanmax(d32)),
                            "pixels_over_1e-4": int(np.sum(d32 > 1e-4))}}
     for form, (coh, count, pairs) in gpu_maps.items():
+        band, allp, ph = band_of[form]
         cc = coh[y0:y0 + size, x0:x0 + size].reshape(-1)
         kc = count[y0:y0 + size, x0:x0 + size].reshape(-1)
         pc = pairs[:, y0:y0 + size, x0:x0 + size].reshape(pairs.shape[0], size * size).T
+        adopt = allp[:, :, x0:x0 + size].reshape(allp.shape[0], -1).T.astype(np.float64)
+        phc = ph[:, :, x0:x0 + size].reshape(n, -1)
+        z64 = np.empty((size * size, n), complex)
+        zmag = np.empty((size * size, n))
+        c64, p64, near, _ = fp64.run(slc_c, ocount, mask, sd, pc_c, size, size, band=band,
+                                     adopt=adopt if sd["bias"] else None, z_out=z64,
+                                     zmag_out=zmag)
         d = np.abs(cc - c64)
         dp = np.abs(pc - p64)
+        both = np.isfinite(phc.T) & np.isfinite(z64.real)
+        clear = np.isfinite(zmag) & ((zmag > 2e-3) | (zmag < 0.5e-3))
+        dph = np.abs(np.angle(np.exp(1j * np.where(both, phc.T, 0.0)) *
+                              np.conj(np.where(both, z64, 1.0))))[both & clear]
         worst = int(np.nanargmax(d))
-        off = np.where(near == 0, d, np.nan)
+        over = int(np.sum(d > 1e-4))
         out[form] = {
             "max_abs_dgamma": float(np.nanmax(d)),
             "rms_dgamma": float(np.sqrt(np.nanmean(d ** 2))),
-            "pixels_over_1e-4": int(np.sum(d > 1e-4)),
-            "max_abs_dgamma_off_bias_threshold": float(np.nanmax(off)),
+            "pixels_over_1e-4": over,
+            "gate_1e-4": bool(over == 0 and np.array_equal(np.isnan(cc), np.isnan(c64))),
             "max_abs_dpair": float(np.nanmax(dp)) if dp.size else None,
+            "rms_dpair": float(np.sqrt(np.nanmean(dp ** 2))) if dp.size else None,
+            "max_abs_dphase_rad": float(dph.max()) if dph.size else None,
+            "rms_dphase_rad": float(np.sqrt(np.mean(dph ** 2))) if dph.size else None,
+            "phases_compared": int(dph.size),
+            "bias_band_m2c": band,
+            "entries_in_band": int(near.sum()),
             "nan_mask_equal": bool(np.array_equal(np.isnan(cc), np.isnan(c64))),
             "shp_count_mismatches": int(np.sum(kc != ocount)),
             "worst": dict(pixel_conditioning(fp64, slc_c, ocount, mask, sd, pc_c,
@@ -236,6 +332,8 @@ def main():
                     help="rows of the CPU baseline band (default 384, 64, 4 for "
                          "configs 2, 3, 5)")
     ap.add_argument("--crop", type=int, default=256)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph-replay rate (tcoh_set_graph)")
     ap.add_argument("--no-region", action="store_true",
                     help="skip the host-to-host rate through tcoh_run_region")
     ap.add_argument("--config", type=int, default=2, choices=[2, 3, 5],
@@ -321,7 +419,14 @@ def main():
 
     forms = {}
     maps = {}
+    band_of = {}
+    band_same = {}
+    graph = {}
     order = [args.precision] + ([] if args.no_plain else ["plain"])
+    crop = min(args.crop, rows, cols)
+    cy0 = max(0, rows // 2 - crop // 2)
+    cx0 = max(0, cols // 2 - crop // 2)
+    emi = s_opts["estimator"] == "emi"
     for form in order:
         steps = args.steps if form == args.precision else max(2, args.steps // 2)
         ms, st_ms, nl = run_form(tcoh, s, pcols, bufs, rows, cols, form, steps,
@@ -330,6 +435,20 @@ def main():
         maps[form] = (coh.cpu().numpy(), count.cpu().numpy(), pairs.cpu().numpy())
         log("%s: %.2f ms/step, stages prep/shp/link ms %s" %
             (form, ms, np.round(st_ms, 3).tolist()))
+        if form == args.precision and not args.no_graph:
+            gms, same = run_graph(tcoh, s, pcols, bufs, rows, cols, form, steps, torch,
+                                  maps[form])
+            graph = {"ms_per_step": gms, "value": rows * cols / (gms * 1e-3) / 1e6,
+                     "maps_equal_eager": same, "steps": steps}
+            log("%s graph replay: %.2f ms/step, maps equal: %s" % (form, gms, same))
+        if crop > 0:
+            bc, bn, bp, bph = band_maps(tcoh, torch, s_opts, n, slc, amp, valid, pcols,
+                                        cy0, crop, cols, form)
+            band_same[form] = bool(
+                np.array_equal(bc.view(np.uint32),
+                               maps[form][0][cy0:cy0 + crop].view(np.uint32)) and
+                np.array_equal(bn, maps[form][1][cy0:cy0 + crop]))
+            band_of[form] = (M2C_BAND["emi" if emi else form], bp, bph)
 
     ms, st_ms, nl, steps = forms[args.precision]
     npix = rows * cols
@@ -350,9 +469,17 @@ def main():
     flop_cov = float(np.sum(4.0 * n * (n + 1) * looks))
     flop_link = est * flops_eigen(n)
     flop_f64 = est * float(n) ** 3 if s_opts["estimator"] == "emi" else 0.0
-    t_bound = 3.0 * flop_cov / PEAK_F16_MFMA + flop_link / PEAK_FP32 + flop_f64 / PEAK_FP64
-    kname = "k_link %s (split-16-bit MFMA covariance + FP32 linking%s)" % (
-        args.precision, ", FP64 Cholesky" if flop_f64 else "")
+    if emi:
+        # EMI: covariance sums in FP64 on the vector ALUs (every form), the
+        # linking in FP32, the Cholesky of |T| in FP64
+        t_bound = flop_cov / PEAK_FP64 + flop_link / PEAK_FP32 + flop_f64 / PEAK_FP64
+        kname = "k_link EMI (FP64 covariance sums + FP64 Cholesky + FP32 linking) " \
+                "and its two queue kernels"
+        bound = "valu"
+    else:
+        t_bound = 3.0 * flop_cov / PEAK_F16_MFMA + flop_link / PEAK_FP32
+        kname = "k_link %s (split-16-bit MFMA covariance + FP32 linking)" % args.precision
+        bound = "mfma"
     flop = flop_cov + flop_link + flop_f64
     avg_launch_s = st_ms[dom] * 1e-3 / max(1.0, nl[dom])
     achieved = flop / max(1.0, nl[dom]) / avg_launch_s
@@ -360,7 +487,24 @@ def main():
     # the HBM traffic was measured (rocprofv3 counters) on this workload only
     traffic_measured = (args.config == 2 and args.precision == "f16"
                         and rows == 2048 and cols == 2048)
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+    # Matrix-core work the mixed form executes: per estimated pixel,
+    # ceil(L/32) chunks of 32 candidates, each 12 v_mfma_f32_16x16x32 per
+    # upper 16 x 16 date block pair (hi.hi, hi.lo, lo.hi of Re Re^T, Im Im^T,
+    # Im Re^T, Re Im^T), 16*16*32*2 flop each.
+    nbk = (16 if n <= 16 else (32 if n <= 32 else 64)) // 16
+    mfma_per_chunk = 12 * nbk * (nbk + 1) // 2
+    mfma = None
+    if not emi:
+        executed = float(np.sum(np.ceil(looks / 32.0))) * mfma_per_chunk * 16 * 16 * 32 * 2
+        mfma = {"executed_flop_per_launch": executed / max(1.0, nl[dom]),
+                "instructions_per_launch": executed / (16 * 16 * 32 * 2) / max(1.0, nl[dom]),
+                "util_of_2.5PF_over_k_link": executed / max(1.0, nl[dom]) / avg_launch_s /
+                PEAK_F16_MFMA,
+                "algorithmic_covariance_share": 3.0 * flop_cov / executed,
+                "note": "executed MFMA flop rate over the whole fused kernel's time "
+                        "(covariance and linking); the rocprofv3 SQ_INSTS_MFMA count of "
+                        "the same kernel is in profiles/"}
+    roofline = {"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": TRAFFIC_F16_N30 if traffic_measured else None,
                 "traffic_source": ("profiles/k_link_n30_2048x2048_kernel_stats.md "
@@ -368,6 +512,8 @@ def main():
                                    "calibrated on a 1 GiB copy)") if traffic_measured else None,
                 "kernel": kname, "avg_launch_ms": avg_launch_s * 1e3,
                 "launches_per_step": float(nl[dom])}
+    if mfma and args.precision != "plain":
+        roofline["mfma"] = mfma
 
     out = {
         "metric": {2: METRIC, 3: METRIC_EMI, 5: METRIC_GEO}[args.config],
@@ -391,6 +537,9 @@ def main():
                    "stage_ms": dict(zip(stage_names, np.round(st_ms, 3).tolist()))},
         "roofline": roofline,
     }
+    if graph:
+        out["graph"] = dict(graph, path="tcoh_set_graph: the strip replayed as one "
+                                        "captured HIP graph per step")
     if "plain" in forms:
         pms, pst, _, psteps = forms["plain"]
         out["plain"] = {"value": npix / (pms * 1e-3) / 1e6, "ms_per_step": pms,
@@ -425,20 +574,19 @@ def main():
             (dt * 1e3, npix / dt / 1e6, same))
         del host_planes, hi, hq, hh
 
-    if args.crop > 0:
+    if crop > 0:
         t0 = time.perf_counter()
-        y0 = max(0, rows // 2 - args.crop // 2)
-        x0 = max(0, cols // 2 - args.crop // 2)
-        size = min(args.crop, rows, cols)
         out["accuracy"] = crop_accuracy(tcoh, cref, fp64, slc.cpu().numpy(),
                                         amp.cpu().numpy(), valid.cpu().numpy(), pcols,
-                                        n, s_opts, y0, x0, size, maps)
-        out["accuracy"]["crop"] = [y0, x0, size]
+                                        n, s_opts, cy0, cx0, crop, maps, band_of)
+        out["accuracy"]["crop"] = [cy0, cx0, crop]
+        for form, same in band_same.items():
+            out["accuracy"][form]["band_rerun_equals_timed_run"] = same
         log("accuracy crop in %.1f s" % (time.perf_counter() - t0))
 
     if not args.no_cpu:
         band = min(cpu_rows, rows)
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         so = cref.settings(n, **s_opts)
         pr_b = band + wa - 1
         sl = slc[:pr_b].cpu().numpy()
@@ -450,7 +598,9 @@ def main():
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {
             "value": band * cols / dt / 1e6, "unit": "Mpixel/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "threads_rule": "every core this process may run on (sched_getaffinity), "
+                            "at most OMP_NUM_THREADS when that is set",
             "sample": "%d x %d px band of the same frame (%.1f s), FP32 C restatement "
                       "of the reference kernel, OpenMP over rows" % (band, cols, dt)}
     print(json.dumps(out))

@@ -135,7 +135,12 @@ def lib():
     L.ocl_close.argtypes = [vp]
     L.tcoh_run_device.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp,
                                   vp, vp, vp]
+    L.tcoh_run_device_phases.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                         vp, vp, vp, vp, vp]
+    L.tcoh_run_phases.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp,
+                                  vp, vp]
     L.tcoh_set_precision.argtypes = [vp, C.c_int]
+    L.tcoh_set_graph.argtypes = [vp, C.c_int]
     L.tcoh_set_timing.argtypes = [vp, C.c_int]
     L.tcoh_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
     L.tcoh_stage_launches.argtypes = [vp, C.POINTER(C.c_int)]
@@ -272,6 +277,11 @@ class Backend:
     def set_timing(self, on):
         lib().tcoh_set_timing(self.h, 1 if on else 0)
 
+    def set_graph(self, on):
+        """Graph mode: strips replayed as one captured HIP graph."""
+        lib().tcoh_set_graph(self.h, 1 if on else 0)
+        _raise_if_error()
+
     def stage_times(self):
         """Last run's (prep, shp, covariance, eigen) device ms and launch
         counts (timing must be on)."""
@@ -281,8 +291,10 @@ class Backend:
         lib().tcoh_stage_launches(self.h, nl)
         return list(ms), list(nl)
 
-    def run(self, slc, amp, valid, padded_rows, rows, cols):
-        """ocl_run on host arrays: (coh, count, pairs (npairs, rows*cols))."""
+    def run(self, slc, amp, valid, padded_rows, rows, cols, phases=False):
+        """ocl_run on host arrays: (coh, count, pairs (npairs, rows*cols)),
+        and with phases (tcoh_run_phases) also the linked phases (N,
+        rows*cols) in radians."""
         s = self.settings
         slc = np.ascontiguousarray(slc, np.float32)
         amp = np.ascontiguousarray(amp, np.float32)
@@ -294,16 +306,29 @@ class Backend:
         coh = np.empty(rows * cols, np.float32)
         count = np.empty(rows * cols, np.int32)
         pairs = np.empty((s.npairs, rows * cols), np.float32) if s.npairs else None
+        if phases:
+            ph = np.empty((s.ndates, rows * cols), np.float32)
+            lib().tcoh_run_phases(self.h, _ptr(slc), _ptr(amp), _ptr(valid), padded_rows,
+                                  rows, cols, _ptr(coh), _ptr(count), _ptr(pairs),
+                                  _ptr(ph))
+            _raise_if_error()
+            return coh, count, pairs, ph
         lib().ocl_run(self.h, _ptr(slc), _ptr(amp), _ptr(valid), padded_rows, rows,
                       cols, _ptr(coh), _ptr(count), _ptr(pairs))
         _raise_if_error()
         return coh, count, pairs
 
     def run_device(self, slc_ptr, amp_ptr, valid_ptr, padded_rows, rows, cols,
-                   coh_ptr, count_ptr, pairs_ptr=None, stream=None):
-        """tcoh_run_device on device pointers (ints), asynchronous."""
-        lib().tcoh_run_device(self.h, slc_ptr, amp_ptr, valid_ptr, padded_rows,
-                              rows, cols, coh_ptr, count_ptr, pairs_ptr, stream)
+                   coh_ptr, count_ptr, pairs_ptr=None, stream=None, phases_ptr=None):
+        """tcoh_run_device (tcoh_run_device_phases with phases_ptr) on device
+        pointers (ints), asynchronous."""
+        if phases_ptr:
+            lib().tcoh_run_device_phases(self.h, slc_ptr, amp_ptr, valid_ptr,
+                                         padded_rows, rows, cols, coh_ptr, count_ptr,
+                                         pairs_ptr, phases_ptr, stream)
+        else:
+            lib().tcoh_run_device(self.h, slc_ptr, amp_ptr, valid_ptr, padded_rows,
+                                  rows, cols, coh_ptr, count_ptr, pairs_ptr, stream)
         _raise_if_error()
 
     def prepare_strip_device(self, i_ptr, q_ptr, nrows, ncols, row0, padded_rows,

@@ -158,6 +158,40 @@ struct StageTimer {
     }
 };
 
+/* What a captured strip depends on: a graph is replayed only for the same
+   buffers, shape, stream and arithmetic. */
+struct GraphKey {
+    const void *slc, *amp, *valid, *coh, *count, *pairs, *phase;
+    int padded_rows, rows, cols, precision;
+    hipStream_t stream;
+
+    bool operator==(const GraphKey &k) const
+    {
+        return slc == k.slc && amp == k.amp && valid == k.valid && coh == k.coh &&
+               count == k.count && pairs == k.pairs && phase == k.phase &&
+               padded_rows == k.padded_rows && rows == k.rows && cols == k.cols &&
+               precision == k.precision && stream == k.stream;
+    }
+};
+
+struct GraphCache {
+    bool valid = false;
+    GraphKey key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void release()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        valid = false;
+    }
+};
+
 struct ocl {
     int device;
     hipStream_t stream;
@@ -169,12 +203,12 @@ struct ocl {
     int stage_launches[4];
     StageTimer timer;
     /* ocl_run's transfer buffers */
-    DevBuf slc, amp, valid, coh, count, pairs;
+    DevBuf slc, amp, valid, coh, count, pairs, phase;
     /* scratch */
-    DevBuf ampT, lstat, mask, adtab, hl, absmax;
-    /* EMI pixels deferred to the Jacobi queue kernel: T, pixel, count */
-    DevBuf qt, qpix, qn;
-    int adtab_n;
+    DevBuf ampT, lstat, mask, adtab, hl, ex;
+    /* EMI band state (EmiQueue): T, eigenvectors and inversion kind of the
+       band's pixels, the two queues and their counts */
+    DevBuf qt, qz, qhow, qpix, qn, qnext;
     /* the phase grid of tcoh_prepare_strip_device / tcoh_run_region */
     DevBuf grid;
     /* tcoh_run_region: double-buffered date planes and outputs on the
@@ -186,6 +220,9 @@ struct ocl {
     hipStream_t copy = nullptr;
     hipEvent_t ev_h2d[2], ev_prep[2], ev_comp[2], ev_d2h[2];
     bool region_ready = false;
+    /* graph mode (tcoh_set_graph) and the captured strip */
+    bool graph = false;
+    struct GraphCache *gcache = nullptr;
 };
 
 static int env_precision()
@@ -203,13 +240,12 @@ static int env_precision()
 
 static bool eig_supported(int n);
 static hipError_t launch_link(const KParams &kp, const float2 *slc,
-                              const uint2 *hl, const uint32_t *mask,
+                              const uint2 *hl, const short *ex, const uint32_t *mask,
                               const int *count, float *coh, float *pairs,
-                              int row0, int nrows, int precision, float2 *qt,
-                              int *qpix, int *qn, hipStream_t s);
-static hipError_t launch_emi_queue(const KParams &kp, const float2 *qt,
-                                   const int *qpix, int qcount, float *coh,
-                                   hipStream_t s);
+                              int row0, int nrows, int precision,
+                              const EmiQueue &q, hipStream_t s);
+static hipError_t launch_emi_queue(const KParams &kp, const EmiQueue &q, int pbase,
+                                   int cap, float *coh, hipStream_t s);
 
 extern "C" void ocl_list_devices(void)
 {
@@ -273,6 +309,8 @@ extern "C" struct ocl *ocl_open(int platform, int device,
 
             struct ocl *o = new struct ocl();
             hipDeviceProp_t pr;
+
+            o->gcache = new GraphCache();
             size_t fr = 0, tot = 0;
 
             o->device = device;
@@ -280,7 +318,6 @@ extern "C" struct ocl *ocl_open(int platform, int device,
             o->padded_cols = padded_cols;
             o->precision = env_precision();
             o->timing = 0;
-            o->adtab_n = 0;
             check(hipSetDevice(device), "hipSetDevice");
             check(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking),
                   "hipStreamCreate");
@@ -338,34 +375,44 @@ extern "C" void tcoh_report_error(const char *msg) { report(msg); }
 
 extern "C" void tcoh_clear_error(void) { g_last_error.clear(); }
 
-/* q(j, m) = (float)(2Nm - jN)^2 / (float)(j(2N - j)) with the two roundings
-   of tcoh.cl:71-72. */
-#pragma clang fp contract(off)
-static void fill_adtab(int n, float *q)
-{
-    for (int j = 0; j < 2 * n; j++)
-        for (int m = 0; m <= n; m++) {
-            if (j == 0) {
-                q[m] = 0.0f;
-                continue;
-            }
-            const float num = (float)(2 * n * m - j * n);
-            const float den = (float)(j * (2 * n - j));
-            volatile float sq = num * num;
+/* Device scratch of one strip: where each buffer the kernels use besides
+   the caller's inputs and outputs lives. */
+struct StripScratch {
+    float *ampT = nullptr;
+    float2 *lstat = nullptr;
+    uint32_t *mask = nullptr;
+    float *adtab = nullptr;
+    uint32_t *hl = nullptr;
+    short *ex = nullptr;
+    EmiQueue q = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
 
-            q[j * (n + 1) + m] = sq / den;
-        }
-}
+/* Sizes of the strip's scratch buffers (bytes; 0: not used), in the order
+   of StripScratch. */
+enum { SC_AMPT, SC_LSTAT, SC_MASK, SC_ADTAB, SC_HL, SC_EX, SC_QT, SC_QZ, SC_QHOW, SC_QPIX,
+       SC_QNEXT, SC_QN, SC_COUNT };
+struct ScratchPlan {
+    size_t bytes[SC_COUNT];
+    int chunk, cap; /* rows per link launch, pixels per band */
+    bool mixed, bf16;
 
-static void run_device(struct ocl *o, const void *slc, const void *amp,
-                       const void *valid, int padded_rows, int rows, int cols,
-                       void *coh, void *count, void *pairs, hipStream_t st)
+    size_t total(size_t align) const
+    {
+        size_t t = 0;
+
+        for (size_t b : bytes)
+            t += (b + align - 1) / align * align;
+        return t;
+    }
+};
+
+/* The strip's settings as the kernels take them, after the strip checks. */
+static KParams strip_params(const struct ocl *o, int padded_rows, int rows, int cols,
+                            const void *pairs, void *phase)
 {
     const struct settings &s = o->s;
     KParams kp;
 
-    if (rows <= 0 || cols <= 0)
-        return;
     if (cols + s.win_rg - 1 != o->padded_cols)
         fail("Strip of %d columns does not match the padded width %d", cols,
              o->padded_cols);
@@ -377,8 +424,6 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
              o->padded_cols);
     if (s.npairs && !pairs)
         fail("Pair coherences requested without an output buffer");
-    check(hipSetDevice(o->device), "hipSetDevice");
-
     kp.n = s.ndates;
     kp.wa = s.win_az;
     kp.wr = s.win_rg;
@@ -398,117 +443,142 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
     kp.ad_sigma = kernel_float(s.ad_sigma);
     kp.ad_crit = kernel_float(s.ad_crit);
     kp.tlog_crit = kernel_float(s.tlog_crit);
-
-    const size_t npad = (size_t)kp.npad, nout = (size_t)rows * cols;
-    const int n = kp.n;
-
-    if (npad * shp_bucket(n) > (size_t)INT32_MAX)
-        fail("Strip of %zu pixels too large for %d dates (at most %d pixels; "
+    kp.phase = (float *)phase;
+    if ((size_t)kp.npad * shp_bucket(kp.n) > (size_t)INT32_MAX)
+        fail("Strip of %d pixels too large for %d dates (at most %d pixels; "
              "use strips within ocl_open's max_alloc)",
-             npad, n, INT32_MAX / shp_bucket(n));
-    o->ampT.ensure(npad * shp_bucket(n) * sizeof(float), "hipMalloc(ampT)");
-    o->mask.ensure(nout * kp.words * sizeof(uint32_t), "hipMalloc(mask)");
-    if (kp.test == 2)
-        o->lstat.ensure(npad * sizeof(float2), "hipMalloc(lstat)");
-    if (kp.test == 1 && o->adtab_n != n) {
-        float *q = (float *)malloc(sizeof(float) * 2 * n * (n + 1));
+             kp.npad, kp.n, INT32_MAX / shp_bucket(kp.n));
+    return kp;
+}
 
-        fill_adtab(n, q);
-        o->adtab.ensure(sizeof(float) * 2 * n * (n + 1), "hipMalloc(adtab)");
-        check(hipMemcpy(o->adtab.p, q, sizeof(float) * 2 * n * (n + 1),
-                        hipMemcpyHostToDevice),
-              "hipMemcpy(adtab)");
-        free(q);
-        o->adtab_n = n;
+static ScratchPlan scratch_plan(const KParams &kp, int precision)
+{
+    ScratchPlan pl;
+    const size_t npad = (size_t)kp.npad, nout = (size_t)kp.rows * kp.cols;
+    const size_t n = kp.n, nb = shp_bucket(kp.n), nt = n * (n - 1) / 2;
+
+    for (size_t &b : pl.bytes)
+        b = 0;
+    /* EMI takes the plain form with FP64 sums whatever the setting */
+    pl.mixed = precision != TCOH_PLAIN && !kp.emi;
+    pl.bf16 = precision == TCOH_MIXED_BF16;
+    pl.chunk = std::max(1, (1 << 20) / kp.cols);
+    pl.cap = std::min(pl.chunk, kp.rows) * kp.cols;
+    pl.bytes[SC_AMPT] = npad * nb * sizeof(float);
+    pl.bytes[SC_MASK] = nout * kp.words * sizeof(uint32_t);
+    if (kp.test == 2)
+        pl.bytes[SC_LSTAT] = npad * sizeof(float2);
+    if (kp.test == 1)
+        pl.bytes[SC_ADTAB] = sizeof(float) * 2 * n * (n + 1);
+    if (pl.mixed) {
+        /* samples as (hi, lo) 16-bit pairs, one zero pixel appended for the
+           padding of candidate lists; FP16 with each pixel's exponent */
+        pl.bytes[SC_HL] = (npad + 1) * nb * 2 * sizeof(uint32_t);
+        if (!pl.bf16)
+            pl.bytes[SC_EX] = (npad + 1) * sizeof(short);
     }
+    if (kp.emi) {
+        const size_t cap = (size_t)pl.cap;
+
+        pl.bytes[SC_QT] = cap * nt * sizeof(float2);
+        pl.bytes[SC_QZ] = cap * n * sizeof(float2);
+        pl.bytes[SC_QHOW] = cap;
+        pl.bytes[SC_QPIX] = cap * sizeof(int);
+        pl.bytes[SC_QNEXT] = cap * sizeof(int);
+        pl.bytes[SC_QN] = 2 * sizeof(int);
+    }
+    return pl;
+}
+
+static StripScratch scratch_pointers(void *const p[SC_COUNT])
+{
+    StripScratch sc;
+
+    sc.ampT = (float *)p[SC_AMPT];
+    sc.lstat = (float2 *)p[SC_LSTAT];
+    sc.mask = (uint32_t *)p[SC_MASK];
+    sc.adtab = (float *)p[SC_ADTAB];
+    sc.hl = (uint32_t *)p[SC_HL];
+    sc.ex = (short *)p[SC_EX];
+    sc.q = {(float2 *)p[SC_QT], (float2 *)p[SC_QZ], (unsigned char *)p[SC_QHOW],
+            (int *)p[SC_QPIX], (int *)p[SC_QN], (int *)p[SC_QNEXT]};
+    return sc;
+}
+
+/* The handle's own scratch, grown to the plan (buffers only grow). */
+static StripScratch handle_scratch(struct ocl *o, const ScratchPlan &pl)
+{
+    DevBuf *bufs[SC_COUNT] = {&o->ampT, &o->lstat, &o->mask, &o->adtab, &o->hl, &o->ex,
+                              &o->qt,   &o->qz,    &o->qhow, &o->qpix,  &o->qnext, &o->qn};
+    void *p[SC_COUNT];
+
+    for (int k = 0; k < SC_COUNT; k++) {
+        p[k] = nullptr;
+        if (pl.bytes[k]) {
+            bufs[k]->ensure(pl.bytes[k], "hipMalloc(strip scratch)");
+            p[k] = bufs[k]->p;
+        }
+    }
+    return scratch_pointers(p);
+}
+
+/* One strip's launches on st into the given scratch. Nothing here
+   allocates, copies from the host or waits; timing (events around the
+   stages) only when asked. */
+static void enqueue_kernels(struct ocl *o, const KParams &kp, const ScratchPlan &pl,
+                            const StripScratch &sc, const void *slc, const void *amp,
+                            const void *valid, void *coh, void *count, void *pairs,
+                            hipStream_t st, bool timing)
+{
     StageTimer &tm = o->timer;
-    const bool timing = o->timing != 0;
     auto mark = [&](int stage, bool start) {
         if (timing)
             tm.mark(stage, st, start);
     };
+    const int rows = kp.rows, cols = kp.cols;
 
-    const bool mixed = o->precision != TCOH_PLAIN;
-    const int bf16 = o->precision == TCOH_MIXED_BF16;
-    /* The FP16 split's scale comes from the strip's largest amplitude,
-       which the preparation kernel folds into this word: no separate pass
-       over the samples and no host round trip. */
-    unsigned int *absmax = nullptr;
-
-    if (mixed && !bf16) {
-        o->absmax.ensure(sizeof(unsigned int), "hipMalloc(absmax)");
-        absmax = o->absmax.as<unsigned int>();
-    }
     if (timing)
         tm.begin();
     mark(0, true);
-    check(launch_prep(kp, (const float *)amp, o->ampT.as<float>(),
-                      (const unsigned char *)valid, o->lstat.as<float2>(), absmax, st),
+    if (kp.test == 1)
+        check(launch_adtab(kp.n, sc.adtab, st), "AD table kernel");
+    check(launch_prep(kp, (const float *)amp, sc.ampT, (const unsigned char *)valid,
+                      sc.lstat, st),
           "prep kernel");
     mark(0, false);
     mark(1, true);
-    check(launch_shp(kp, o->ampT.as<float>(), (const unsigned char *)valid,
-                     o->lstat.as<float2>(), o->adtab.as<float>(),
-                     o->mask.as<uint32_t>(), (int *)count, st),
+    check(launch_shp(kp, sc.ampT, (const unsigned char *)valid, sc.lstat, sc.adtab,
+                     sc.mask, (int *)count, st),
           "shp kernel");
     mark(1, false);
-
-    if (mixed) {
-        /* Samples as (hi, lo) 16-bit pairs, one zero pixel appended for
-           the padding of candidate lists. */
-        o->hl.ensure((npad + 1) * shp_bucket(n) * 2 * sizeof(uint32_t), "hipMalloc(hl)");
+    if (pl.mixed) {
         /* counted with the preparation stage */
         mark(0, true);
-        check(launch_split(kp, (const float2 *)slc, o->hl.as<uint32_t>(), bf16, absmax,
-                           st),
+        check(launch_split(kp, (const float2 *)slc, sc.hl, pl.bf16 ? nullptr : sc.ex,
+                           pl.bf16, st),
               "split kernel");
         mark(0, false);
     }
 
     /* Fused covariance + linking, in launches of about a million output
-       pixels (each well under the GPU watchdog, ocl.c:22-23). EMI at
-       depths up to 32 defers the pixels that need the Jacobi
-       pseudo-inverse to a queue and links them in a second launch, four
-       to a wavefront. */
-    const int chunk = std::max(1, (1 << 20) / cols);
-    const bool queue = kp.emi && n <= 32;
-    const size_t nt = (size_t)n * (n - 1) / 2;
-
-    if (queue) {
-        const size_t cap = (size_t)std::min(chunk, rows) * cols;
-
-        o->qt.ensure(cap * nt * sizeof(float2), "hipMalloc(EMI queue)");
-        o->qpix.ensure(cap * sizeof(int), "hipMalloc(EMI queue pixels)");
-        o->qn.ensure(sizeof(int), "hipMalloc(EMI queue count)");
-    }
-    for (int r0 = 0; r0 < rows; r0 += chunk) {
-        const int nr = std::min(chunk, rows - r0);
+       pixels (each well under the GPU watchdog, ocl.c:22-23). EMI defers
+       the pixels whose |T| fails the Cholesky test to queue 1 and links
+       them in more launches (Gauss-Jordan, the Jacobi for what that
+       leaves, the FP64 refinement); those kernels read the queue counts on
+       the device, so nothing waits on the host. */
+    for (int r0 = 0; r0 < rows; r0 += pl.chunk) {
+        const int nr = std::min(pl.chunk, rows - r0);
 
         mark(2, true);
-        if (queue)
-            check(hipMemsetAsync(o->qn.p, 0, sizeof(int), st), "hipMemset(queue)");
-        check(launch_link(kp, (const float2 *)slc,
-                          (const uint2 *)o->hl.as<uint32_t>(),
-                          o->mask.as<uint32_t>(), (const int *)count,
-                          (float *)coh, (float *)pairs, r0, nr, o->precision,
-                          queue ? o->qt.as<float2>() : nullptr,
-                          queue ? o->qpix.as<int>() : nullptr,
-                          queue ? o->qn.as<int>() : nullptr, st),
+        if (kp.emi)
+            check(launch_zero_words((uint32_t *)sc.q.n, 2, st), "queue count kernel");
+        check(launch_link(kp, (const float2 *)slc, (const uint2 *)sc.hl, sc.ex, sc.mask,
+                          (const int *)count, (float *)coh, (float *)pairs, r0, nr,
+                          o->precision, sc.q, st),
               "link kernel");
-        if (queue) {
-            int qcount = 0;
-
-            check(hipMemcpyAsync(&qcount, o->qn.p, sizeof(int),
-                                 hipMemcpyDeviceToHost, st),
-                  "hipMemcpy(queue count)");
-            check(hipStreamSynchronize(st), "hipStreamSynchronize");
-            if ((size_t)qcount > (size_t)nr * cols)
-                fail("EMI queue of %d pixels for a band of %zu", qcount,
-                     (size_t)nr * cols);
-            check(launch_emi_queue(kp, o->qt.as<float2>(), o->qpix.as<int>(),
-                                   qcount, (float *)coh, st),
-                  "EMI queue kernel");
-        }
+        if (kp.emi)
+            check(launch_emi_queue(kp, sc.q, r0 * cols, nr * cols, (float *)coh, st),
+                  "EMI queue kernels");
         mark(2, false);
     }
     if (timing) {
@@ -528,6 +598,83 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
     }
 }
 
+/* One strip's launches on st, in the handle's scratch (allocated here
+   when it has to grow). With capture set the stream is being captured
+   into a graph: the eager run of the same call has allocated everything,
+   and nothing is timed. */
+static void enqueue_strip(struct ocl *o, const void *slc, const void *amp,
+                          const void *valid, int padded_rows, int rows, int cols,
+                          void *coh, void *count, void *pairs, void *phase, hipStream_t st,
+                          bool capture)
+{
+    if (rows <= 0 || cols <= 0)
+        return;
+    const KParams kp = strip_params(o, padded_rows, rows, cols, pairs, phase);
+
+    check(hipSetDevice(o->device), "hipSetDevice");
+    const ScratchPlan pl = scratch_plan(kp, o->precision);
+    const StripScratch sc = handle_scratch(o, pl);
+
+    enqueue_kernels(o, kp, pl, sc, slc, amp, valid, coh, count, pairs, st,
+                    o->timing != 0 && !capture);
+}
+
+/* A strip on the device, eagerly or (graph mode, timing off) as a HIP
+   graph: the first call with a new key runs eagerly, then captures the
+   same launches into a graph that later calls with that key replay with
+   one hipGraphLaunch. Every launch of a strip is a kernel (the queue
+   counters are cleared by one), and none waits on the host. */
+static void run_device(struct ocl *o, const void *slc, const void *amp,
+                       const void *valid, int padded_rows, int rows, int cols,
+                       void *coh, void *count, void *pairs, void *phase, hipStream_t st)
+{
+    GraphCache &gc = *o->gcache;
+    const GraphKey key = {slc,         amp,  valid, coh,  count,        pairs, phase,
+                          padded_rows, rows, cols,  o->precision, st};
+
+    if (!o->graph || o->timing) {
+        enqueue_strip(o, slc, amp, valid, padded_rows, rows, cols, coh, count, pairs, phase,
+                      st, false);
+        return;
+    }
+    if (gc.valid && gc.key == key) {
+        check(hipSetDevice(o->device), "hipSetDevice");
+        check(hipGraphLaunch(gc.exec, st), "hipGraphLaunch");
+        return;
+    }
+    enqueue_strip(o, slc, amp, valid, padded_rows, rows, cols, coh, count, pairs, phase, st,
+                  false);
+    gc.release();
+    check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed),
+          "hipStreamBeginCapture");
+    try {
+        enqueue_strip(o, slc, amp, valid, padded_rows, rows, cols, coh, count, pairs, phase,
+                      st, true);
+    }
+    catch (...) {
+        hipGraph_t g = nullptr;
+
+        (void)hipStreamEndCapture(st, &g);
+        if (g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    check(hipStreamEndCapture(st, &gc.graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0),
+          "hipGraphInstantiate");
+    gc.key = key;
+    gc.valid = true;
+}
+
+extern "C" void tcoh_set_graph(struct ocl *o, int on)
+{
+    guarded_void([&] {
+        o->graph = on != 0;
+        if (!o->graph)
+            o->gcache->release();
+    });
+}
+
 extern "C" void tcoh_run_device(struct ocl *o, const void *slc,
                                 const void *amp, const void *valid,
                                 int padded_rows, int rows, int cols, void *coh,
@@ -535,15 +682,30 @@ extern "C" void tcoh_run_device(struct ocl *o, const void *slc,
 {
     guarded_void([&] {
         run_device(o, slc, amp, valid, padded_rows, rows, cols, coh, count,
-                   pairs, stream ? (hipStream_t)stream : o->stream);
+                   pairs, nullptr, stream ? (hipStream_t)stream : o->stream);
     });
 }
 
-extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
-                        const unsigned char *valid, int padded_rows, int rows,
-                        int cols, float *coh, int *count, float *pairs)
+extern "C" void tcoh_run_device_phases(struct ocl *o, const void *slc,
+                                       const void *amp, const void *valid,
+                                       int padded_rows, int rows, int cols,
+                                       void *coh, void *count, void *pairs,
+                                       void *phases, void *stream)
 {
     guarded_void([&] {
+        if (!phases)
+            fail("Linked phases requested without an output buffer");
+        run_device(o, slc, amp, valid, padded_rows, rows, cols, coh, count,
+                   pairs, phases, stream ? (hipStream_t)stream : o->stream);
+    });
+}
+
+/* ocl_run's transfers around run_device, with the optional phases. */
+static void host_run(struct ocl *o, const float *slc, const float *amp,
+                     const unsigned char *valid, int padded_rows, int rows,
+                     int cols, float *coh, int *count, float *pairs, float *phases)
+{
+    {
         const size_t npad = (size_t)padded_rows * o->padded_cols;
         const size_t nout = (size_t)rows * cols;
         const size_t n = o->s.ndates;
@@ -566,8 +728,10 @@ extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
         check(hipMemcpyAsync(o->valid.p, valid, npad, hipMemcpyHostToDevice,
                              st),
               "hipMemcpy(valid)");
+        if (phases)
+            o->phase.ensure(n * nout * sizeof(float), "hipMalloc(phases)");
         run_device(o, o->slc.p, o->amp.p, o->valid.p, padded_rows, rows, cols,
-                   o->coh.p, o->count.p, o->pairs.p, st);
+                   o->coh.p, o->count.p, o->pairs.p, phases ? o->phase.p : nullptr, st);
         check(hipMemcpyAsync(coh, o->coh.p, nout * sizeof(float),
                              hipMemcpyDeviceToHost, st),
               "hipMemcpy(coh)");
@@ -579,7 +743,32 @@ extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
         check(hipMemcpyAsync(count, o->count.p, nout * sizeof(int),
                              hipMemcpyDeviceToHost, st),
               "hipMemcpy(count)");
+        if (phases)
+            check(hipMemcpyAsync(phases, o->phase.p, n * nout * sizeof(float),
+                                 hipMemcpyDeviceToHost, st),
+                  "hipMemcpy(phases)");
         check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+}
+
+extern "C" void ocl_run(struct ocl *o, const float *slc, const float *amp,
+                        const unsigned char *valid, int padded_rows, int rows,
+                        int cols, float *coh, int *count, float *pairs)
+{
+    guarded_void([&] {
+        host_run(o, slc, amp, valid, padded_rows, rows, cols, coh, count, pairs, nullptr);
+    });
+}
+
+extern "C" void tcoh_run_phases(struct ocl *o, const float *slc, const float *amp,
+                                const unsigned char *valid, int padded_rows, int rows,
+                                int cols, float *coh, int *count, float *pairs,
+                                float *phases)
+{
+    guarded_void([&] {
+        if (!phases)
+            fail("Linked phases requested without an output buffer");
+        host_run(o, slc, amp, valid, padded_rows, rows, cols, coh, count, pairs, phases);
     });
 }
 
@@ -849,8 +1038,8 @@ extern "C" void tcoh_run_region(struct ocl *o, int nrows, int ncols,
             float *dout = o->rout[b].as<float>();
             const size_t px = (size_t)rs * ncols;
 
-            run_device(o, o->rslc.p, o->ramp.p, o->rvalid.p, prow, rs, ncols, dout,
-                       dout + px, np ? dout + 2 * px : nullptr, cs);
+            enqueue_strip(o, o->rslc.p, o->ramp.p, o->rvalid.p, prow, rs, ncols, dout,
+                          dout + px, np ? dout + 2 * px : nullptr, nullptr, cs, false);
             check(hipEventRecord(o->ev_comp[b], cs), "hipEventRecord");
             /* 4. outputs back on the copy stream */
             check(hipStreamWaitEvent(o->copy, o->ev_comp[b], 0), "hipStreamWaitEvent");
@@ -876,9 +1065,9 @@ extern "C" void ocl_close(struct ocl *o)
         (void)hipSetDevice(o->device);
         (void)hipStreamSynchronize(o->stream);
         DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
-                          &o->count, &o->pairs, &o->ampT,  &o->lstat,
+                          &o->count, &o->pairs, &o->phase, &o->ampT,  &o->lstat,
                           &o->mask, &o->adtab, &o->hl,
-                          &o->absmax, &o->qt, &o->qpix, &o->qn,
+                          &o->ex, &o->qt, &o->qz, &o->qhow, &o->qpix, &o->qn, &o->qnext,
                           &o->grid, &o->rin[0], &o->rin[1],
                           &o->rhgt[0], &o->rhgt[1], &o->rout[0], &o->rout[1],
                           &o->rslc, &o->ramp, &o->rvalid};
@@ -902,6 +1091,8 @@ extern "C" void ocl_close(struct ocl *o)
             (void)hipStreamDestroy(o->copy);
         }
         o->timer.release();
+        o->gcache->release();
+        delete o->gcache;
         (void)hipStreamDestroy(o->stream);
         delete o;
     });
@@ -911,14 +1102,14 @@ extern "C" void ocl_close(struct ocl *o)
    to X(3) X(4) ... for the depths the build compiled. */
 #define TCOH_EIG_CASE(NN)                                                    \
     case NN:                                                                 \
-        return launch_link_n<NN>(kp, slc, hl, mask, count, coh, pairs, row0, \
-                                 nrows, precision, qt, qpix, qn, s);
+        return launch_link_n<NN>(kp, slc, hl, ex, mask, count, coh, pairs,   \
+                                 row0, nrows, precision, q, s);
 
 static hipError_t launch_link(const KParams &kp, const float2 *slc,
-                              const uint2 *hl, const uint32_t *mask,
+                              const uint2 *hl, const short *ex, const uint32_t *mask,
                               const int *count, float *coh, float *pairs,
-                              int row0, int nrows, int precision, float2 *qt,
-                              int *qpix, int *qn, hipStream_t s)
+                              int row0, int nrows, int precision,
+                              const EmiQueue &q, hipStream_t s)
 {
     switch (kp.n) {
         TCOH_EIG_FOR_EACH(TCOH_EIG_CASE)
@@ -929,11 +1120,10 @@ static hipError_t launch_link(const KParams &kp, const float2 *slc,
 
 #define TCOH_QUEUE_CASE(NN)                                                  \
     case NN:                                                                 \
-        return launch_emi_queue_n<NN>(kp, qt, qpix, qcount, coh, s);
+        return launch_emi_queue_n<NN>(kp, q, pbase, cap, coh, s);
 
-static hipError_t launch_emi_queue(const KParams &kp, const float2 *qt,
-                                   const int *qpix, int qcount, float *coh,
-                                   hipStream_t s)
+static hipError_t launch_emi_queue(const KParams &kp, const EmiQueue &q, int pbase,
+                                   int cap, float *coh, hipStream_t s)
 {
     switch (kp.n) {
         TCOH_EIG_FOR_EACH(TCOH_QUEUE_CASE)

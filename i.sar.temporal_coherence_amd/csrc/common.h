@@ -21,6 +21,8 @@ struct KParams {
     int npairs, pairs_all;
     int words;             /* 32-bit mask words per output pixel */
     float ad_norm, ad_sigma, ad_crit, tlog_crit;
+    float *phase;          /* optional: linked phase of every date, planar
+                              [N][rows][cols] (radians, NaN where gamma is) */
 };
 
 /* Largest window the SHP mask supports (candidates per pixel). */
@@ -382,14 +384,18 @@ __host__ __device__ __forceinline__ int upper_index(int n, int i, int j)
 /* Kernel launchers (defined in the .hip files). Each returns a HIP error
    code; s is the handle's stream. */
 hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
-                       const unsigned char *valid, float2 *lstat,
-                       unsigned int *absmax, hipStream_t s);
+                       const unsigned char *valid, float2 *lstat, hipStream_t s);
 hipError_t launch_shp(const KParams &kp, const float *ampT,
                       const unsigned char *valid, const float2 *lstat,
                       const float *adtab, uint32_t *mask, int *count,
                       hipStream_t s);
-hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl,
-                        int bf16, const unsigned int *absmax, hipStream_t s);
+/* The mixed forms' samples as hi | lo << 16 words (uint2 per sample, NB
+   date slots per pixel, one zero pixel appended); FP16 with each pixel
+   scaled by its own power of two, whose exponent goes to ex (one short a
+   pixel, SPLIT_EX_ZERO for a pixel without non-zero finite samples). */
+#define SPLIT_EX_ZERO (-32768)
+hipError_t launch_split(const KParams &kp, const float2 *slc, uint32_t *hl, short *ex,
+                        int bf16, hipStream_t s);
 /* Strip preparation (main.c:52-127) on the device. */
 struct StripArgs {
     const float *i, *q;       /* date planes [n][nrows][ncols], NaN null */
@@ -407,20 +413,41 @@ struct StripArgs {
     double hx[3], h0;         /* grid heights, height without elevation */
 };
 hipError_t launch_strip(const StripArgs &sa, hipStream_t s);
+/* EMI state of one band (band pixel bp = output pixel - first pixel of
+   the band; capacity: the band's pixels). k_link stores T of every
+   estimated pixel (NT float2 each at t) and appends the pixels whose |T|
+   fails the Cholesky test to queue 1 (pix, n[0] entries); the first queue
+   kernel inverts theirs by Gauss-Jordan and appends those beyond its
+   condition bound to queue 2 (next, n[1] entries) for the Jacobi. Every
+   linking kernel leaves the FP32 eigenvector (N float2 at z) and how the
+   pixel was inverted (how: 0 not refined, 1 Cholesky, 2 Gauss-Jordan) for
+   the FP64 refinement kernel. n is cleared before each band. */
+struct EmiQueue {
+    float2 *t;
+    float2 *z;
+    unsigned char *how;
+    int *pix;
+    int *n;
+    int *next;
+};
 /* Fused covariance + linking of output rows row0 .. row0+nrows-1
-   (precision: 0 plain, 1 split FP16, 2 split BF16), one instantiation
-   per stack depth N. With EMI and a queue (qt non-null, depths up to 32),
-   pixels needing the Jacobi pseudo-inverse are appended to qt / qpix
-   (count in *qn) instead of linked. */
+   (precision: 0 plain, 1 split FP16, 2 split BF16; EMI always plain with
+   FP64 sums), one instantiation per stack depth N. EMI appends the pixels
+   whose |T| fails the Cholesky test to queue 1 instead of linking them. */
 template <int N>
 hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
-                         const uint32_t *mask, const int *count, float *coh,
-                         float *pairs, int row0, int nrows, int precision,
-                         float2 *qt, int *qpix, int *qn, hipStream_t s);
-/* EMI pixels deferred by launch_link_n (queue qt, qpix of qcount entries),
-   PPW to a wavefront; depths up to 32. */
+                         const short *ex, const uint32_t *mask, const int *count,
+                         float *coh, float *pairs, int row0, int nrows, int precision,
+                         const EmiQueue &q, hipStream_t s);
+/* The EMI kernels of a band after launch_link_n on the same stream (the
+   two queue kernels, then the FP64 refinement): pbase is the band's first
+   output pixel, cap its pixels; they read the queue counts on the device. */
 template <int N>
-hipError_t launch_emi_queue_n(const KParams &kp, const float2 *qt, const int *qpix,
-                              int qcount, float *coh, hipStream_t s);
+hipError_t launch_emi_queue_n(const KParams &kp, const EmiQueue &q, int pbase, int cap,
+                              float *coh, hipStream_t s);
+/* The AD test's term table (2N(N+1) floats) on the device. */
+hipError_t launch_adtab(int n, float *q, hipStream_t s);
+/* Clear n 32-bit words (a kernel, not a memset: capturable in a graph). */
+hipError_t launch_zero_words(uint32_t *p, int n, hipStream_t s);
 
 #endif

@@ -10,12 +10,18 @@
  *   blocks of B x B; lane l owns upper block l and keeps its B*B complex
  *   sums in registers; per accepted candidate (dy-major, the reference's
  *   summation order) it loads the B row and B column samples it needs.
+ *   EMI takes this form with FP64 sums, whatever the precision setting:
+ *   |T|^-1 o T amplifies the rounding of FP32 sums (up to 9e-5 in gamma on
+ *   near-singular |T|), and of the split forms' representation likewise.
  *
  * mixed (cov_mfma_band): the whole wavefront on one pixel at a time, as
  *   real Gram products. With Re, Im the N x L matrices of the L accepted
  *   candidates, C_re = Re Re^T + Im Im^T and C_im = Im Re^T - Re Im^T.
- *   Each sample is split as hi + lo (FP16: x = hi + lo + O(2^-22 x); BF16:
- *   O(2^-18 x)) and every product is accumulated in FP32 from hi.hi +
+ *   Each sample is split as hi + lo (FP16: x = hi + lo + O(2^-22 x), each
+ *   pixel under its own power-of-two scale, which the gather turns into
+ *   the window's common one, exactly, by the candidate's exponent below
+ *   the window's largest; BF16: O(2^-18 x), unscaled) and every product
+ *   is accumulated in FP32 from hi.hi +
  *   hi.lo + lo.hi with v_mfma_f32_16x16x32_{f16,bf16}, for the upper
  *   16 x 16 date blocks only. The accepted candidates are compacted into
  *   LDS lists padded to a multiple of 32 with a zero pixel; A and B
@@ -50,6 +56,28 @@ __device__ __forceinline__ float2 coherence_entry(float cre, float cim,
     return make_float2(re, im);
 }
 
+/* The same in FP64 (EMI), rounded to float at the end. */
+__device__ __forceinline__ float2 coherence_entry(double cre, double cim,
+                                                  double den, int bias,
+                                                  int looks)
+{
+    double re = 0.0, im = 0.0;
+
+    if (den > 0.0) {
+        re = cre / den;
+        im = cim / den;
+        if (bias && looks > 1) {
+            const double m2 = re * re + im * im;
+            const double m2c = (looks * m2 - 1.0) / (looks - 1.0);
+            const double f = (m2 > 0.0 && m2c > 0.0) ? sqrt(m2c / m2) : 0.0;
+
+            re *= f;
+            im *= f;
+        }
+    }
+    return make_float2((float)re, (float)im);
+}
+
 /* Pair plane of T_ij, or -1 if (i, j) is not an output pair. */
 __device__ __forceinline__ int pair_plane(const KParams &kp, int i, int j)
 {
@@ -62,13 +90,15 @@ template <int G> struct PlainShape {
     static constexpr int NBLK = G == 8 ? 3 : (G == 16 ? 5 : 10);
 };
 
-/* Plain form, one pixel per group. diag: LDS scratch of N floats. */
-template <int N, int G>
+/* Plain form, one pixel per group, sums in ACC: float (the reference's
+   arithmetic) or double (EMI, whose |T|^-1 o T amplifies the rounding of
+   FP32 sums). diag: LDS scratch of N ACC values. */
+template <int N, int G, typename ACC>
 __device__ __forceinline__ void cov_plain_group(const KParams &kp,
                                                 const float2 *__restrict__ slc,
                                                 const uint32_t *__restrict__ mask,
                                                 int p, int cnt, bool est,
-                                                bool live, int lg, float *diag,
+                                                bool live, int lg, ACC *diag,
                                                 float2 *tl,
                                                 float *__restrict__ pairs)
 {
@@ -77,7 +107,7 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
     const size_t plane = (size_t)kp.rows * kp.cols;
     const int y = p / kp.cols, x = p - (p / kp.cols) * kp.cols;
     int bi = 0, bj = 0;
-    float cr[B][B], ci[B][B];
+    ACC cr[B][B], ci[B][B];
 
     /* Upper block number lg -> (bi, bj), bi <= bj. */
     {
@@ -96,7 +126,7 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
     for (int r = 0; r < B; r++)
 #pragma unroll
         for (int c = 0; c < B; c++)
-            cr[r][c] = ci[r][c] = 0.0f;
+            cr[r][c] = ci[r][c] = (ACC)0;
 
     if (est && own) {
         const uint32_t *mk = mask + (size_t)p * kp.words;
@@ -118,8 +148,10 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
                 for (int r = 0; r < B; r++)
 #pragma unroll
                     for (int c = 0; c < B; c++) {
-                        cr[r][c] += sr[r].x * sc[c].x + sr[r].y * sc[c].y;
-                        ci[r][c] += sr[r].y * sc[c].x - sr[r].x * sc[c].y;
+                        const ACC ar = sr[r].x, ai = sr[r].y, br = sc[c].x, bi_ = sc[c].y;
+
+                        cr[r][c] += ar * br + ai * bi_;
+                        ci[r][c] += ai * br - ar * bi_;
                     }
             }
         }
@@ -150,8 +182,13 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
                     pairs[(size_t)pidx * plane + p] = __builtin_nanf("");
                 continue;
             }
-            const float2 t = coherence_entry(
-                cr[r][c], ci[r][c], sqrt_fast(diag[i]) * sqrt_fast(diag[j]), kp.bias, cnt);
+            ACC den;
+
+            if constexpr (sizeof(ACC) == 8)
+                den = sqrt(diag[i]) * sqrt(diag[j]);
+            else
+                den = sqrt_fast(diag[i]) * sqrt_fast(diag[j]);
+            const float2 t = coherence_entry(cr[r][c], ci[r][c], den, kp.bias, cnt);
 
             tl[upper_index(N, i, j)] = t;
             if (pidx >= 0)
@@ -200,7 +237,32 @@ __device__ __forceinline__ Opnd make_opnd(const uint2 (&v)[8])
 
 /* Candidate lists of the mixed form: the strip pixel index of every
    accepted candidate, padded to a multiple of 32 with the zero pixel
-   (index npad). */
+   (index npad); in the FP16 form the top bits hold the candidate's
+   exponent below the window's largest (strip pixels stay below 2^27,
+   since npad * NB <= INT32_MAX with NB >= 16). */
+#define LIST_SHIFT_BITS 27
+#define LIST_SHIFT_MAX 31
+#define LIST_INDEX_MASK 0x7FFFFFFu
+
+/* FP16 bits of 2^-s, 0 <= s <= 31 (subnormal from s = 15, zero above
+   24), in both halves of a word. */
+__device__ __forceinline__ uint32_t f16_pow2_neg_pair(int s)
+{
+    const uint32_t b = s <= 14 ? (uint32_t)(15 - s) << 10
+                               : (s <= 24 ? 1u << (24 - s) : 0u);
+
+    return b | (b << 16);
+}
+
+/* Both FP16 halves of w times those of f (v_pk_mul_f16: exact for a power
+   of two unless the product is subnormal, then rounded to nearest). */
+__device__ __forceinline__ uint32_t pk_mul_f16(uint32_t w, uint32_t f)
+{
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(half2_t, w) *
+                                            __builtin_bit_cast(half2_t, f));
+}
 __host__ __device__ __forceinline__ int mfma_list_stride(int kw)
 {
     return (kw + 31) & ~31;
@@ -225,6 +287,7 @@ __host__ __device__ __forceinline__ int mfma_scratch_bytes(int ppw, int kw,
 template <int N, int NB, int PPW, int BF16>
 __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                                               const uint2 *__restrict__ hl,
+                                              const short *__restrict__ ex,
                                               const uint32_t *__restrict__ mask,
                                               const int *__restrict__ count,
                                               int y, int x0,
@@ -273,7 +336,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         }
         int *lq = list + q * lstride;
         const int origin = y * kp.w + x0 + q;
-        int base = 0;
+        int base = 0, emax = SPLIT_EX_ZERO;
 
         for (int it = 0; it * 64 < kw; it++) {
             /* The words are uniform: read them into scalar registers, so
@@ -296,9 +359,26 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                 const int b = it * 64 + lane;
                 const int dy = (int)(((float)b + 0.5f) * inv_wr);
 
-                lq[base + rank] = origin + dy * kp.w + (b - dy * kp.wr);
+                const int cand = origin + dy * kp.w + (b - dy * kp.wr);
+
+                lq[base + rank] = cand;
+                if (!BF16)
+                    emax = max(emax, (int)ex[cand]);
             }
             base += __popcll(m64);
+        }
+        if (!BF16) {
+            /* FP16: each entry also carries its pixel's exponent below the
+               window's largest accepted one (LIST_SHIFT_BITS), by which
+               step() rescales its hi and lo parts */
+            emax = gmaxi<WAVE>(emax);
+            wave_sync();
+            for (int t = lane; t < base; t += WAVE) {
+                const int cand = lq[t];
+                const int sh = min(emax - (int)ex[cand], LIST_SHIFT_MAX);
+
+                lq[t] = (int)((unsigned)cand | ((unsigned)sh << LIST_SHIFT_BITS));
+            }
         }
         const int lpad = (base + 31) & ~31;
 
@@ -351,7 +431,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const uint2 *row = hl + (size_t)(unsigned)idx[e] * NB + i16;
+            const uint2 *row = hl + (size_t)((unsigned)idx[e] & LIST_INDEX_MASK) * NB + i16;
 
 #pragma unroll
             for (int bk = 0; bk < NBK; bk++)
@@ -369,10 +449,41 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     auto step = [&](int c, const uint2(&use)[NBK][8]) {
         Opnd op[NBK];
         uint4_t nrh[NBK], nrl[NBK];
+        uint2 sv[NBK][8];
+
+        if (!BF16) {
+            /* the window's common scale: each candidate's hi and lo times
+               2^-(its exponent below the largest), from the list entries
+               of this chunk (read again here: the gathers were issued
+               chunks ago) */
+            int cq, off;
+
+            locate(c, cq, off);
+            const int4 *lp = reinterpret_cast<const int4 *>(list + cq * lstride + off * 32 + g * 8);
+            const int4 i0 = lp[0], i1 = lp[1];
+            const int ent[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const uint32_t f = f16_pow2_neg_pair((int)((unsigned)ent[e] >> LIST_SHIFT_BITS));
+
+#pragma unroll
+                for (int bk = 0; bk < NBK; bk++)
+                    sv[bk][e] = make_uint2(pk_mul_f16(use[bk][e].x, f),
+                                           pk_mul_f16(use[bk][e].y, f));
+            }
+        }
+        else {
+#pragma unroll
+            for (int bk = 0; bk < NBK; bk++)
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    sv[bk][e] = use[bk][e];
+        }
 
 #pragma unroll
         for (int bk = 0; bk < NBK; bk++) {
-            op[bk] = make_opnd(use[bk]);
+            op[bk] = make_opnd(sv[bk]);
             nrh[bk] = op[bk].rh ^ 0x80008000u;
             nrl[bk] = op[bk].rl ^ 0x80008000u;
         }

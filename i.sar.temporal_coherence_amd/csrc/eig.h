@@ -12,10 +12,12 @@
  *  EVD (tcoh.cl:649-656)            eigenvector of the largest eigenvalue
  *  EMI (tcoh.cl:657-694)            |T|^-1 by Cholesky in FP64 (pivot
  *                                   floor 1e-5 as the reference); else the
- *                                   eigen pseudo-inverse of a cyclic
- *                                   Jacobi in FP32 (relative floor 1e-6);
- *                                   then the smallest eigenvalue of
- *                                   |T|^-1 o T
+ *                                   eigen pseudo-inverse (relative floor
+ *                                   1e-6), in FP64: a Gauss-Jordan
+ *                                   inverse where a condition bound shows
+ *                                   that the floor drops nothing, else a
+ *                                   cyclic Jacobi; then the smallest
+ *                                   eigenvalue of |T|^-1 o T
  *  eigenvector (tcoh.cl:201-444)    Householder reduction to a real
  *                                   tridiagonal (zhetd2, lower), then the
  *                                   eigenvalue of the wanted rank by
@@ -67,15 +69,14 @@ template <int N> struct EigLds {
     float2 z[N];
 };
 
-/* Cholesky factor (FP64, lower, column-major packed, with the reciprocal
-   diagonal) or, in the fallback, the Jacobi state (FP32): the rotations of
-   a round, the matrix A and the eigenvectors V (NE x NE, NE = N rounded up
-   to even; the padding row and column stay zero in A). */
-template <int N> struct JacobiShape {
-    static constexpr int NE = (N + 1) & ~1;
-    static constexpr int NP = NE / 2;
-};
-
+/* Inversion state of one pixel's |T| (EMI), FP64, one area per group; each
+   kernel reserves only the member it uses:
+   ch   Cholesky factor (lower, column-major packed), reciprocal diagonal,
+        the pivot flag and, above 32 dates, W = L^-1
+   gj   Gauss-Jordan exchange: the pivot row of the current step, the
+        pivot row of every step and |T|^-1 (float, both triangles)
+   jac  cyclic Jacobi eigendecomposition: A, V (row-major N x N) and the
+        reciprocal eigenvalues */
 template <int N> union __attribute__((aligned(16))) EmiLds {
     struct {
         double lc[N * (N + 1) / 2];
@@ -84,12 +85,14 @@ template <int N> union __attribute__((aligned(16))) EmiLds {
         double w[N > 32 ? N * (N + 1) / 2 : 1]; /* L^-1 (N > 32) */
     } ch;
     struct {
-        float4 rot[JacobiShape<N>::NP];
-        int4 pq[JacobiShape<N>::NP]; /* p, q, rotation applied */
-        float2 dd[JacobiShape<N>::NP];
-        float il[JacobiShape<N>::NE];
-        float a[JacobiShape<N>::NE * JacobiShape<N>::NE];
-        float v[JacobiShape<N>::NE * JacobiShape<N>::NE];
+        double prow[N];
+        int pk[N];
+        float inv[N * N];
+    } gj;
+    struct {
+        double a[N * N];
+        double v[N * N];
+        double il[N];
     } jac;
 };
 
@@ -746,14 +749,14 @@ __device__ __forceinline__ float2 load_t(const float2 *__restrict__ tp, int r,
     return make_float2(t.x, -t.y);
 }
 
-/* |T|^-1 into inv (owned columns) for EMI (tcoh.cl:657-687). Returns
-   false, with inv untouched, when the Cholesky pivot test fails and the
-   caller defers the Jacobi pseudo-inverse (fallback false) to a queue of
-   such pixels processed four to a wavefront. */
+/* |T|^-1 into inv (owned columns) for EMI by Cholesky in FP64 with the
+   reference's pivot floor (tcoh.cl:446-492). Returns false, with inv
+   untouched, when a pivot is at or below the floor: the reference then
+   takes the eigen pseudo-inverse (tcoh.cl:657-687), which the caller
+   leaves to the queue kernels (gj_inverse, jacobi_pinv). */
 template <int N, int G, int CPL>
-__device__ __forceinline__ bool abs_inverse(const float2 *__restrict__ tp, bool est,
-                            EmiLds<N> &em, int lg, float (&inv)[CPL][N],
-                            bool fallback)
+__device__ __forceinline__ bool chol_inverse(const float2 *__restrict__ tp, bool est,
+                                             EmiLds<N> &em, int lg, float (&inv)[CPL][N])
 {
     bool ok = true;
 
@@ -948,263 +951,246 @@ __device__ __forceinline__ bool abs_inverse(const float2 *__restrict__ tp, bool 
     if (ok)
         return true;
     }
-    if (!fallback)
-        return false;
+    return false;
+}
 
-    /* Not safely positive definite: pseudo-inverse from a Jacobi
-       eigendecomposition of |T| in FP32 with the reference's rotations,
-       stopping rule and sweep limit (tcoh.cl:97-199, 663-687), in the
-       parallel (round-robin tournament) ordering, systolic: lane i < NP
-       holds the two columns of A and of V at tournament positions i and
-       NE-1-i (a zero index pads odd N). A is kept in position coordinates
-       (rows permuted with the columns), so every row index below is
-       static. Per round each lane rotates its own pair's columns, all
-       pairs' row rotations are applied from a broadcast, and the columns
-       move one tournament position (wavefront shifts). |T| is real, so
-       the phase step of the complex rotation is a sign. */
-    typedef JacobiShape<N> J;
-    constexpr int NE = J::NE, NP = J::NP;
-    float *a = em.jac.a, *v = em.jac.v;
-    const int uL = lg, uR = NE - 1 - lg; /* this lane's positions */
-    const bool act = lg < NP;
-    float cL[NE], cR[NE], vL[N], vR[N];
-    int iL = act ? uL : 0, iR = act ? uR : 0; /* matrix index at each */
+template <int G> __device__ __forceinline__ double gmaxd(double v)
+{
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, G));
+    return v;
+}
+
+template <int G> __device__ __forceinline__ int gmini(int v)
+{
+    return -gmaxi<G>(-v);
+}
+
+/* Condition bound below which the eigen pseudo-inverse of |T| with the
+   reference's relative floor (tcoh.cl:669-677, EIG_FLOOR) is the inverse:
+   for symmetric A, min |lambda| = 1 / ||A^-1||_2 >= 1 / ||A^-1||_F and
+   max |lambda| <= ||A||_F, so ||A||_F ||A^-1||_F <= 1 / EIG_FLOOR drops no
+   eigenvalue. Compared squared. */
+#define PINV_IS_INV_KAPPA2 1.0e12
+
+/* |T|^-1 into inv for a pixel whose Cholesky pivot test failed (|T| not
+   safely positive definite, usually indefinite), by Gauss-Jordan exchange
+   steps in FP64 with partial pivoting. Lane lg keeps rows r = lg + m G of
+   the tableau in registers. Step k exchanges column k with the unused row
+   p_k of largest |a_rk| (ties to the lowest row); that row is broadcast
+   through LDS, and the tableau stays in place, so after N steps row p_i
+   holds row i of the inverse in the column order p_0 .. p_{N-1}, which the
+   owners scatter to LDS. Returns true when ||A||_F ||A^-1||_F is within
+   the bound above (the reference's pseudo-inverse is then this inverse);
+   false, for the eigen pseudo-inverse of jacobi_pinv, otherwise (also for
+   a singular A, whose exchange divides by zero). */
+template <int N, int G, int CPL>
+__device__ __forceinline__ bool gj_inverse(const float2 *__restrict__ tp, bool est,
+                                           EmiLds<N> &em, int lg, float (&inv)[CPL][N])
+{
+    double a[CPL][N];
+    bool used[CPL];
+    int step[CPL];
+    double fa = 0.0;
 
 #pragma unroll
-    for (int r = 0; r < NE; r++) {
-        float2 tl_ = make_float2(r == iL ? 1.f : 0.f, 0.f);
-        float2 tr_ = make_float2(r == iR ? 1.f : 0.f, 0.f);
+    for (int m = 0; m < CPL; m++) {
+        const int r = lg + m * G;
 
-        if (est && r < N && iL < N && act)
-            tl_ = load_t<N>(tp, r, iL);
-        if (est && r < N && iR < N && act)
-            tr_ = load_t<N>(tp, r, iR);
-        cL[r] = (r < N && iL < N) ? hypotf(tl_.x, tl_.y) : 0.0f;
-        cR[r] = (r < N && iR < N) ? hypotf(tr_.x, tr_.y) : 0.0f;
-    }
+        used[m] = r >= N;
+        step[m] = 0;
 #pragma unroll
-    for (int r = 0; r < N; r++) {
-        vL[r] = r == iL ? 1.0f : 0.0f;
-        vR[r] = r == iR ? 1.0f : 0.0f;
-    }
-    /* Dynamic position reads as select chains over static registers. */
-    auto at = [&](const float(&x)[NE], int u) {
-        float o = 0.0f;
+        for (int c = 0; c < N; c++) {
+            float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
 
-#pragma unroll
-        for (int r = 0; r < NE; r++)
-            o = r == u ? x[r] : o;
-        return o;
-    };
-    for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
-        float off = 0.0f, dg = 0.0f;
-
-        /* off-diagonal squares summed directly (never as a difference of
-           two large sums): each pair p < q appears in two columns */
-        if (act) {
-#pragma unroll
-            for (int r = 0; r < NE; r++) {
-                off += r == uL ? 0.0f : cL[r] * cL[r];
-                off += r == uR ? 0.0f : cR[r] * cR[r];
-            }
-            const float dl = at(cL, uL), dr = at(cR, uR);
-
-            dg = fmaf(dl, dl, dr * dr);
+            if (est && r < N)
+                t = load_t<N>(tp, r, c);
+            a[m][c] = r < N ? (double)hypotf(t.x, t.y) : 0.0;
+            fa = fma(a[m][c], a[m][c], fa);
         }
-        off = 0.5f * gsum<G>(off);
-        dg = gsum<G>(dg);
-        if (off <= 1.0e-14f * dg)
-            break;
-        for (int round = 0; round < NE - 1; round++) {
-            /* Rotation of this lane's pair in position terms:
-               new col uL = j.x col uL + j.z col uR,
-               new col uR = j.y col uL + j.w col uR. */
-            float4 jr = make_float4(1.f, 0.f, 0.f, 1.f);
-            float dpp_ = 0.f, dqq_ = 0.f;
-            bool rot = false, lp = iL < iR; /* L holds the lower index */
-
-            if (act) {
-                const float aLL = at(cL, uL), aRR = at(cR, uR), aLR = at(cL, uR);
-                const float app = lp ? aLL : aRR, aqq = lp ? aRR : aLL;
-                const float r = fabsf(aLR);
-
-                if (r != 0.0f) {
-                    const float sg = aLR < 0.0f ? -1.0f : 1.0f;
-                    const float theta = (aqq - app) / (2.0f * r);
-                    float t;
-
-                    if (fabsf(theta) > 1.0e18f)
-                        t = 0.5f / theta;
-                    else
-                        t = (theta >= 0.0f ? 1.0f : -1.0f) /
-                            (fabsf(theta) + sqrtf(theta * theta + 1.0f));
-                    const float cs = 1.0f / sqrtf(t * t + 1.0f), sn = t * cs;
-
-                    /* new col p = cs p - sn sg q, new col q = sn p + cs sg q */
-                    jr = lp ? make_float4(cs, sn, -sn * sg, cs * sg)
-                            : make_float4(cs * sg, -sn * sg, sn, cs);
-                    dpp_ = app - t * r;
-                    dqq_ = aqq + t * r;
-                    rot = true;
-                }
-                em.jac.rot[lg] = jr;
-            }
-            /* A J and V J on the own columns */
+    }
+    fa = gsumd<G>(fa);
 #pragma unroll
-            for (int r = 0; r < NE; r++) {
-                const float x = cL[r], y = cR[r];
-
-                cL[r] = fmaf(jr.x, x, jr.z * y);
-                cR[r] = fmaf(jr.y, x, jr.w * y);
-            }
-#pragma unroll
-            for (int r = 0; r < N; r++) {
-                const float x = vL[r], y = vR[r];
-
-                vL[r] = fmaf(jr.x, x, jr.z * y);
-                vR[r] = fmaf(jr.y, x, jr.w * y);
-            }
-            wave_sync();
-            /* J^T A: rows (k, NE-1-k) of every pair k */
-#pragma unroll
-            for (int k = 0; k < NP; k++) {
-                const float4 jk = em.jac.rot[k];
-                const int u = k, w = NE - 1 - k;
-                const float xl = cL[u], yl = cL[w], xr = cR[u], yr = cR[w];
-
-                cL[u] = fmaf(jk.x, xl, jk.z * yl);
-                cL[w] = fmaf(jk.y, xl, jk.w * yl);
-                cR[u] = fmaf(jk.x, xr, jk.z * yr);
-                cR[w] = fmaf(jk.y, xr, jk.w * yr);
-            }
-            /* the own 2 x 2 block exactly, as the reference sets it */
-            if (rot) {
-                const float dl = lp ? dpp_ : dqq_, dr = lp ? dqq_ : dpp_;
+    for (int k = 0; k < N; k++) {
+        double bv = -1.0;
+        int br = N;
 
 #pragma unroll
-                for (int r = 0; r < NE; r++) {
-                    cL[r] = r == uL ? dl : (r == uR ? 0.0f : cL[r]);
-                    cR[r] = r == uR ? dr : (r == uL ? 0.0f : cR[r]);
-                }
+        for (int m = 0; m < CPL; m++) {
+            const double v = fabs(a[m][k]);
+
+            if (!used[m] && v > bv) {
+                bv = v;
+                br = lg + m * G;
             }
-            wave_sync();
-            /* Next round: position u -> u + 1 on 1 .. NE-1 (NE-1 -> 1),
-               position 0 fixed; rows of A follow inside each column, and
-               the columns move between lanes: L of lane i -> L of lane
-               i+1 (lane NP-1: its own R); R of lane i -> R of lane i-1
-               (lane 0: L of lane 1). */
-#pragma unroll
-            for (int X = 0; X < 2; X++) {
-                float(&c)[NE] = X ? cR : cL;
-                const float last = c[NE - 1];
+        }
+        const double mx = gmaxd<G>(bv);
+        const int p = gmini<G>(bv == mx ? br : N);
 
 #pragma unroll
-                for (int r = NE - 1; r >= 2; r--)
-                    c[r] = c[r - 1];
-                if (NE > 1)
-                    c[1] = last;
-            }
-            auto shr = [](float x) {
-                return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-                    0, __builtin_bit_cast(int, x), 0x138, 0xF, 0xF, false));
-            };
-            auto shl = [](float x) {
-                return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-                    0, __builtin_bit_cast(int, x), 0x130, 0xF, 0xF, false));
-            };
-            auto shri = [](int x) {
-                return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false);
-            };
-            auto shli = [](int x) {
-                return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, false);
-            };
+        for (int m = 0; m < CPL; m++)
+            if (lg + m * G == p) {
 #pragma unroll
-            for (int r = 0; r < NE; r++) {
-                const float fromL = shr(cL[r]), fromR = shr(cR[r]), nextR = shl(cR[r]);
-                const float l = cL[r];
-
-                cL[r] = lg == 0 ? l : (lg == 1 ? fromR : fromL);
-                cR[r] = lg == NP - 1 ? l : nextR;
+                for (int c = 0; c < N; c++)
+                    em.gj.prow[c] = a[m][c];
+                used[m] = true;
+                step[m] = k;
             }
+        if (lg == 0)
+            em.gj.pk[k] = p;
+        wave_sync();
+        const double rp = 1.0 / em.gj.prow[k];
+
 #pragma unroll
-            for (int r = 0; r < N; r++) {
-                const float fromL = shr(vL[r]), fromR = shr(vR[r]), nextR = shl(vR[r]);
-                const float l = vL[r];
-
-                vL[r] = lg == 0 ? l : (lg == 1 ? fromR : fromL);
-                vR[r] = lg == NP - 1 ? l : nextR;
+        for (int m = 0; m < CPL; m++) {
+            if (lg + m * G == p) {
+#pragma unroll
+                for (int c = 0; c < N; c++)
+                    a[m][c] = c == k ? rp : -a[m][c] * rp;
             }
-            {
-                const int fromL = shri(iL), fromR = shri(iR), nextR = shli(iR);
-                const int l = iL;
+            else {
+                const double f = a[m][k] * rp;
 
-                iL = lg == 0 ? l : (lg == 1 ? fromR : fromL);
-                iR = lg == NP - 1 ? l : nextR;
+#pragma unroll
+                for (int c = 0; c < N; c++)
+                    a[m][c] = c == k ? f : fma(-f, em.gj.prow[c], a[m][c]);
+            }
+        }
+        wave_sync();
+        TCOH_STEP_FENCE();
+    }
+    double fi = 0.0;
+
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+        const int r = lg + m * G;
+
+        if (r < N) {
+            const int i = step[m];
+
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+                fi = fma(a[m][c], a[m][c], fi);
+                em.gj.inv[i * N + em.gj.pk[c]] = (float)a[m][c];
             }
         }
     }
-    /* eigenvalues and eigenvectors to LDS, by matrix index */
-    if (act) {
-        if (iL < N) {
-            a[iL * NE + iL] = at(cL, uL);
+    fi = gsumd<G>(fi);
+    wave_sync();
 #pragma unroll
-            for (int r = 0; r < N; r++)
-                v[r * NE + iL] = vL[r];
-        }
-        if (iR < N) {
-            a[iR * NE + iR] = at(cR, uR);
+    for (int m = 0; m < CPL; m++) {
+        const int c = lg + m * G;
+
 #pragma unroll
-            for (int r = 0; r < N; r++)
-                v[r * NE + iR] = vR[r];
-        }
+        for (int r = 0; r < N; r++)
+            inv[m][r] = c < N ? em.gj.inv[r * N + c] : 0.0f;
     }
     wave_sync();
+    return fa * fi <= PINV_IS_INV_KAPPA2;
+}
 
-    /* inv = V diag(1 / lambda) V^T with eigenvalues below 1e-6 of the
-       largest magnitude dropped (tcoh.cl:669-677). */
-    float lmax = 0.0f;
+/* Rotations of the Jacobi below are skipped when |a_pq| is within this
+   fraction of sqrt(|a_pp a_qq|) (negligible in FP64). */
+#define JACOBI_TOL 1.0e-17
 
-    for (int k = lg; k < N; k += G)
-        lmax = fmaxf(lmax, fabsf(a[k * NE + k]));
-    lmax = gmax<G>(lmax) * EIG_FLOOR;
-    for (int k = lg; k < N; k += G) {
-        const float lk = a[k * NE + k];
+/* The reference's pseudo-inverse of |T| (tcoh.cl:657-687): its
+   eigendecomposition, then V diag(1 / lambda) V^T with the eigenvalues
+   below EIG_FLOOR of the largest magnitude left out. Here in FP64 by the
+   cyclic Jacobi method (rows p < q in order), run to convergence: a sweep
+   that finds every off-diagonal entry negligible (JACOBI_TOL) ends it,
+   within the reference's sweep limit. Each rotation is the symmetric
+   Schur rotation annihilating a_pq (Golub and Van Loan, Matrix
+   Computations, 4th ed., 8.5.2), applied to the rows and columns k owned
+   by the lanes of the group (k = lg, lg + G, ...) in LDS. Only pixels that
+   gj_inverse could not settle come here (none in the benchmark scenes). */
+template <int N, int G, int CPL>
+__device__ __forceinline__ void jacobi_pinv(const float2 *__restrict__ tp, bool est,
+                                            EmiLds<N> &em, int lg, float (&inv)[CPL][N])
+{
+    double *a = em.jac.a, *v = em.jac.v;
 
-        em.jac.il[k] = fabsf(lk) < lmax ? 0.0f : 1.0f / lk;
-    }
-    wave_sync();
-    /* B = V diag(il) into the dead matrix area, then inv = B V^T row by
-       row, the sum over k rolled (only fallback pixels come here). */
+#pragma unroll 1
     for (int i = lg; i < N * N; i += G) {
-        const int r = i / N, k = i - (i / N) * N;
+        const int r = i / N, c = i - (i / N) * N;
+        const float2 t = est ? load_t<N>(tp, r, c) : make_float2(r == c ? 1.f : 0.f, 0.f);
 
-        a[r * NE + k] = v[r * NE + k] * em.jac.il[k];
+        a[i] = (double)hypotf(t.x, t.y);
+        v[i] = r == c ? 1.0 : 0.0;
     }
     wave_sync();
-#pragma unroll
-    for (int r = 0; r < N; r++) {
-        float sm[CPL];
+#pragma unroll 1
+    for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
+        bool rotated = false;
 
-#pragma unroll
-        for (int m = 0; m < CPL; m++)
-            sm[m] = 0.0f;
-#pragma unroll 2
-        for (int k = 0; k < N; k++) {
-            const float x = v[r * NE + k];
+#pragma unroll 1
+        for (int p = 0; p < N - 1; p++)
+#pragma unroll 1
+            for (int q = p + 1; q < N; q++) {
+                const double apq = a[p * N + q], app = a[p * N + p], aqq = a[q * N + q];
 
-#pragma unroll
-            for (int m = 0; m < CPL; m++) {
-                const int c = lg + m * G;
+                if (!(fabs(apq) > JACOBI_TOL * sqrt(fabs(app) * fabs(aqq))))
+                    continue;
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(fma(tau, tau, 1.0)));
+                const double c = 1.0 / sqrt(fma(t, t, 1.0)), s = t * c;
 
-                sm[m] = fmaf(x, c < N ? a[c * NE + k] : 0.0f, sm[m]);
+                wave_sync();
+#pragma unroll 1
+                for (int k = lg; k < N; k += G) {
+                    if (k != p && k != q) {
+                        const double akp = a[k * N + p], akq = a[k * N + q];
+                        const double np_ = c * akp - s * akq, nq = s * akp + c * akq;
+
+                        a[k * N + p] = np_;
+                        a[p * N + k] = np_;
+                        a[k * N + q] = nq;
+                        a[q * N + k] = nq;
+                    }
+                    const double vkp = v[k * N + p], vkq = v[k * N + q];
+
+                    v[k * N + p] = c * vkp - s * vkq;
+                    v[k * N + q] = s * vkp + c * vkq;
+                }
+                if (lg == 0) {
+                    a[p * N + p] = app - t * apq;
+                    a[q * N + q] = aqq + t * apq;
+                    a[p * N + q] = 0.0;
+                    a[q * N + p] = 0.0;
+                }
+                wave_sync();
+                rotated = true;
             }
-        }
-#pragma unroll
-        for (int m = 0; m < CPL; m++)
-            inv[m][r] = sm[m];
+        if (!rotated)
+            break;
+    }
+    double lmax = 0.0;
+
+    for (int k = 0; k < N; k++)
+        lmax = fmax(lmax, fabs(a[k * N + k]));
+    lmax *= (double)EIG_FLOOR;
+    for (int k = lg; k < N; k += G) {
+        const double lk = a[k * N + k];
+
+        em.jac.il[k] = fabs(lk) < lmax ? 0.0 : 1.0 / lk;
     }
     wave_sync();
-    return true;
+#pragma unroll
+    for (int m = 0; m < CPL; m++) {
+        const int c = lg + m * G;
+
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            double sm = 0.0;
+
+            if (c < N)
+#pragma unroll 1
+                for (int k = 0; k < N; k++)
+                    sm = fma(v[r * N + k] * em.jac.il[k], v[c * N + k], sm);
+            inv[m][r] = (float)sm;
+        }
+    }
+    wave_sync();
 }
 
 #endif

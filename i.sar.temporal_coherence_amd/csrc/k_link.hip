@@ -11,6 +11,7 @@
 
 #include "cov.h"
 #include "link.h"
+#include "refine.h"
 
 template <int N> struct LinkShape {
     static constexpr int G = N <= 16 ? 8 : (N <= 32 ? 16 : 64);
@@ -19,42 +20,71 @@ template <int N> struct LinkShape {
     static constexpr int NB = N <= 16 ? 16 : (N <= 32 ? 32 : 64);
     /* Dynamic LDS of a group: T of each pixel (alive through both stages),
        then a scratch area the covariance and the linking take in turn,
-       then (EMI) the inversion areas. */
+       then (EMI) one inversion area per pixel, of the member of EmiLds the
+       kernel uses (k_link: the Cholesky factor; the queue kernels: the
+       Gauss-Jordan or the Jacobi state). */
     static constexpr int TL_BYTES = (PPW * NT * 8 + 15) & ~15;
     static constexpr int EIG_BYTES = PPW * (int)sizeof(EigLds<N>);
-    static constexpr int EMI_BYTES = PPW * (int)sizeof(EmiLds<N>);
-    /* A group that leaves its Jacobi pixels to the queue uses only the
-       Cholesky part of EmiLds: smaller strides let one more group share a
-       compute unit's LDS (at N=30, 35 KB a group instead of 51 KB: 4 groups
-       per CU, one per SIMD, instead of 3). */
     static constexpr int CH_STRIDE = ((int)sizeof(EmiLds<N>::ch) + 15) & ~15;
-    static constexpr int CH_BYTES = PPW * CH_STRIDE;
+    static constexpr int GJ_STRIDE = ((int)sizeof(EmiLds<N>::gj) + 15) & ~15;
+    static constexpr int JAC_STRIDE = ((int)sizeof(EmiLds<N>::jac) + 15) & ~15;
 };
 
 template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
 {
     typedef LinkShape<N> S;
-    const int cov = precision ? mfma_scratch_bytes(S::PPW, kp.wa * kp.wr, kp.words, S::NB)
-                              : S::PPW * N * 4;
+    /* EMI always takes the plain form with FP64 sums */
+    const int cov = kp.emi ? S::PPW * N * 8
+                    : precision ? mfma_scratch_bytes(S::PPW, kp.wa * kp.wr, kp.words, S::NB)
+                                : S::PPW * N * 4;
 
     return ((cov > S::EIG_BYTES ? cov : S::EIG_BYTES) + 15) & ~15;
 }
 
+/* Append the wavefront's deferred pixels (lane lg == 0 of each group with
+   deferred set) to a queue: returns the slot of the caller's group (valid
+   where deferred). All lanes of the wavefront must be active. */
+template <int G>
+__device__ __forceinline__ int queue_slot(int *qn, bool lead_deferred, int gi)
+{
+    const unsigned long long b = __ballot(lead_deferred);
+    int base = 0;
+
+    if (b == 0ull)
+        return 0;
+    if (threadIdx.x == 0)
+        base = atomicAdd(qn, __popcll(b));
+    base = __shfl(base, 0);
+    return base + __popcll(b & ((1ull << (gi * G)) - 1ull));
+}
+
+/* The linked phases arg(z_c) of the group's pixel p from its phasors in
+   LDS (as link_group leaves them), when the caller asked for them. */
+template <int N, int G>
+__device__ __forceinline__ void store_phases(const KParams &kp, const EigLds<N> &sh, int p,
+                                             int lg, bool linked)
+{
+    if (!kp.phase)
+        return;
+    const size_t plane = (size_t)kp.rows * kp.cols;
+
+    for (int c = lg; c < N; c += G) {
+        const float2 z = sh.z[c];
+
+        kp.phase[(size_t)c * plane + p] = linked ? atan2f(z.y, z.x) : __builtin_nanf("");
+    }
+}
+
 /* cov: 0 plain FP32, 1 split FP16 MFMA, 2 split BF16 MFMA (uniform per
    launch: one code object serves the three forms, so the linking code is
-   compiled once per depth and estimator). */
-/* EMI pixels whose |T| fails the Cholesky pivot test, deferred to
-   k_emi_queue: T (NT float2 each), the output pixel, and the count. */
-struct EmiQueue {
-    float2 *t;
-    int *pix;
-    int *n;
-};
-
+   compiled once per depth and estimator); EMI ignores it (plain, FP64
+   sums). EMI pixels whose |T| fails the Cholesky pivot test go to queue 1
+   (T, NT float2 each, and the output pixel), in place of a result. */
 template <int N, int EMI>
 __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
                                                const uint2 *__restrict__ hl,
+                                               const short *__restrict__ ex,
                                                const uint32_t *__restrict__ mask,
                                                const int *__restrict__ count,
                                                float *__restrict__ coh,
@@ -68,14 +98,9 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     float2 *tl = reinterpret_cast<float2 *>(lds);
     unsigned char *scratch = lds + S::TL_BYTES;
-    /* With a queue, a group whose |T| needs the Jacobi pseudo-inverse
-       leaves it to k_emi_queue, where every group of a wavefront needs it:
-       here one such group would hold up the wavefront's other three. */
-    const bool defer = EMI && q.t != nullptr;
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
-    EmiLds<N> *em = reinterpret_cast<EmiLds<N> *>(
-        scratch + scratch_bytes +
-        (EMI ? gi * (defer ? S::CH_STRIDE : (int)sizeof(EmiLds<N>)) : 0));
+    EmiLds<N> *em = reinterpret_cast<EmiLds<N> *>(scratch + scratch_bytes +
+                                                  (EMI ? gi * S::CH_STRIDE : 0));
     int r, tx;
 
     if (!xcd_tile(blockIdx.x, nrows, (kp.cols + PPW - 1) / PPW, &r, &tx))
@@ -86,123 +111,237 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
     const int cnt = live ? count[p] : -1;
     const bool est = live && cnt >= kp.min_looks;
 
-    if (cov == 0)
-        cov_plain_group<N, G>(kp, slc, mask, p, cnt, est, live, lg,
-                              reinterpret_cast<float *>(scratch) + gi * N,
-                              tl + gi * NT, pairs);
+    if constexpr (EMI)
+        cov_plain_group<N, G, double>(kp, slc, mask, p, cnt, est, live, lg,
+                                      reinterpret_cast<double *>(scratch) + gi * N,
+                                      tl + gi * NT, pairs);
+    else if (cov == 0)
+        cov_plain_group<N, G, float>(kp, slc, mask, p, cnt, est, live, lg,
+                                     reinterpret_cast<float *>(scratch) + gi * N,
+                                     tl + gi * NT, pairs);
     else if (cov == 1)
-        cov_mfma_band<N, NB, PPW, 0>(kp, hl, mask, count, row0 + r, tx * PPW,
+        cov_mfma_band<N, NB, PPW, 0>(kp, hl, ex, mask, count, row0 + r, tx * PPW,
                                      scratch, tl, pairs);
     else
-        cov_mfma_band<N, NB, PPW, 1>(kp, hl, mask, count, row0 + r, tx * PPW,
+        cov_mfma_band<N, NB, PPW, 1>(kp, hl, ex, mask, count, row0 + r, tx * PPW,
                                      scratch, tl, pairs);
     wave_sync();
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(scratch);
+    const int bp = r * kp.cols + (live ? x : 0); /* band pixel */
     bool deferred;
-    const float gam = link_group<N, G, EMI>(kp, tl + gi * NT, est, lg, sh[gi], *em,
-                                            !defer, deferred);
 
-    if (defer) {
-        const unsigned long long b = __ballot(lg == 0 && deferred);
+    if constexpr (EMI) {
+        /* T of the estimated pixel for the queues and the refinement */
+        if (est) {
+            float2 *dst = q.t + (size_t)bp * NT;
 
-        if (b) {
-            int base = 0;
+            for (int t = lg; t < NT; t += G)
+                dst[t] = tl[gi * NT + t];
+        }
+    }
+    /* (only live groups store an eigenvector: a group past the row's end
+       has bp of the row's first pixel) */
+    const float gam = link_group<N, G, EMI, INV_CHOLESKY>(
+        kp, tl + gi * NT, est, lg, sh[gi], *em, deferred,
+        EMI && live ? q.z + (size_t)bp * N : nullptr);
 
-            if (threadIdx.x == 0)
-                base = atomicAdd(q.n, __popcll(b));
-            base = __shfl(base, 0);
-            if (deferred) {
-                const int slot =
-                    base + __popcll(b & ((1ull << (gi * G)) - 1ull));
-                float2 *dst = q.t + (size_t)slot * NT;
+    if constexpr (EMI) {
+        const int slot = queue_slot<G>(q.n, lg == 0 && deferred, gi);
 
-                for (int t = lg; t < NT; t += G)
-                    dst[t] = tl[gi * NT + t];
-                if (lg == 0)
-                    q.pix[slot] = p;
-            }
+        if (lg == 0 && live) {
+            if (deferred)
+                q.pix[slot] = bp;
+            q.how[bp] = (!deferred && !__builtin_isnan(gam)) ? 1 : 0;
         }
     }
     if (live && lg == 0 && !deferred)
         coh[p] = gam;
+    if (live && !deferred)
+        store_phases<N, G>(kp, sh[gi], p, lg, !__builtin_isnan(gam));
 }
 
-/* The deferred EMI pixels, PPW to a wavefront: T read from the queue in
-   place (no LDS copy, so that four groups share a compute unit's LDS), the
-   Jacobi pseudo-inverse of |T| (the Cholesky test fails again, as it did in
-   k_link), then the linking as in k_link. */
+/* Queue 1 (pixels whose |T| failed the Cholesky test), PPW to a
+   wavefront, T read from the band's store in place: |T|^-1 by Gauss-Jordan
+   where the condition bound shows it is the reference's pseudo-inverse,
+   then the linking; the rest go on to queue 2. Persistent: a fixed grid
+   strides over the count that k_link left in q.n[0], so that nothing waits
+   for it on the host. */
 template <int N>
-__global__ __launch_bounds__(64, 1) void k_emi_queue(KParams kp,
-                                                     const float2 *__restrict__ qt,
-                                                     const int *__restrict__ qpix,
-                                                     int qcount,
+__global__ __launch_bounds__(64, 1) void k_emi_queue(KParams kp, EmiQueue q, int pbase,
                                                      float *__restrict__ coh)
 {
     typedef LinkShape<N> S;
     constexpr int G = S::G, PPW = S::PPW, NT = S::NT;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(lds);
-    EmiLds<N> *ems = reinterpret_cast<EmiLds<N> *>(lds + S::EIG_BYTES);
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
-    const int slot = blockIdx.x * PPW + gi;
-    const bool live = slot < qcount;
-    /* a group past the end reads nothing (not estimated) */
-    const float2 *tq = qt + (size_t)(live ? slot : 0) * NT;
-    bool deferred;
-    const float gam = link_group<N, G, 1>(kp, tq, live, lg, sh[gi], ems[gi], true,
-                                          deferred);
+    EmiLds<N> *em = reinterpret_cast<EmiLds<N> *>(lds + S::EIG_BYTES + gi * S::GJ_STRIDE);
+    const int total = q.n[0];
 
-    if (live && lg == 0)
-        coh[qpix[slot]] = gam;
+    for (int base = blockIdx.x * PPW; base < total; base += gridDim.x * PPW) {
+        const int slot = base + gi;
+        const bool live = slot < total;
+        const int bp = live ? q.pix[slot] : 0;
+        bool deferred;
+        const float gam = link_group<N, G, 1, INV_GAUSS_JORDAN>(
+            kp, q.t + (size_t)bp * NT, live, lg, sh[gi], *em, deferred,
+            live ? q.z + (size_t)bp * N : nullptr);
+        const int s2 = queue_slot<G>(q.n + 1, lg == 0 && live && deferred, gi);
+
+        if (live && lg == 0) {
+            if (deferred) {
+                q.next[s2] = bp;
+            }
+            else {
+                coh[pbase + bp] = gam;
+                q.how[bp] = __builtin_isnan(gam) ? 0 : 2;
+            }
+        }
+        if (live && !deferred)
+            store_phases<N, G>(kp, sh[gi], pbase + bp, lg, !__builtin_isnan(gam));
+    }
+}
+
+/* Queue 2 (pixels whose |T| the condition bound leaves to the eigen
+   pseudo-inverse): the FP64 Jacobi pseudo-inverse, then the linking (not
+   refined: how stays 0). */
+template <int N>
+__global__ __launch_bounds__(64, 1) void k_emi_jacobi(KParams kp, EmiQueue q, int pbase,
+                                                      float *__restrict__ coh)
+{
+    typedef LinkShape<N> S;
+    constexpr int G = S::G, PPW = S::PPW, NT = S::NT;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(lds);
+    const int gi = threadIdx.x / G, lg = threadIdx.x % G;
+    EmiLds<N> *em = reinterpret_cast<EmiLds<N> *>(lds + S::EIG_BYTES + gi * S::JAC_STRIDE);
+    const int total = q.n[1];
+
+    for (int base = blockIdx.x * PPW; base < total; base += gridDim.x * PPW) {
+        const int k = base + gi;
+        const bool live = k < total;
+        const int bp = live ? q.next[k] : 0;
+        bool deferred;
+        const float gam = link_group<N, G, 1, INV_JACOBI>(kp, q.t + (size_t)bp * NT, live,
+                                                          lg, sh[gi], *em, deferred);
+
+        if (live && lg == 0)
+            coh[pbase + bp] = gam;
+        if (live)
+            store_phases<N, G>(kp, sh[gi], pbase + bp, lg, !__builtin_isnan(gam));
+    }
+}
+
+/* The FP64 refinement (refine.h) of every linked EMI pixel of the band
+   whose |T|^-1 is the inverse (how 1 or 2): one inverse iteration from its
+   FP32 eigenvector, phasors and gamma in FP64, which replace the FP32
+   results. G = 32 lanes a pixel (two pixels a wavefront) up to 32 dates,
+   64 above; a fixed grid strides over the band. */
+template <int N> struct RefineShape {
+    static constexpr int G = N <= 32 ? 32 : 64;
+    static constexpr int PPW = WAVE / G;
+    static constexpr int STRIDE = ((int)sizeof(RefineLds<N>) + 15) & ~15;
+};
+
+template <int N>
+__global__ __launch_bounds__(64, 1) void k_emi_refine(KParams kp, EmiQueue q, int pbase,
+                                                      int npix, float *__restrict__ coh)
+{
+    typedef RefineShape<N> R;
+    constexpr int G = R::G, PPW = R::PPW, NT = N * (N - 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int gi = threadIdx.x / G, lg = threadIdx.x % G;
+    RefineLds<N> &sh = *reinterpret_cast<RefineLds<N> *>(lds + gi * R::STRIDE);
+    const size_t plane = (size_t)kp.rows * kp.cols;
+
+    for (int base = blockIdx.x * PPW; base < npix; base += gridDim.x * PPW) {
+        const int bp = base + gi;
+        const int how = bp < npix ? q.how[bp] : 0;
+        double g;
+
+        if (how == 0)
+            continue;
+        if (!refine_pixel<N, G>(q.t + (size_t)bp * NT, q.z + (size_t)bp * N, lg, sh,
+                                how == 2, g))
+            continue;
+        if (lg == 0)
+            coh[pbase + bp] = (float)g;
+        if (kp.phase && lg < N) {
+            const double2 u = sh.z[lg];
+
+            kp.phase[(size_t)lg * plane + pbase + bp] = (float)atan2(u.y, u.x);
+        }
+    }
+}
+
+static hipError_t lds_limit(const void *fn, size_t lds)
+{
+    if (lds <= 65536)
+        return hipSuccess;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+/* Groups of the persistent queue kernels: enough for the whole chip, no
+   more than the queue can use. */
+static unsigned queue_grid(int cap, int ppw, int groups)
+{
+    const int need = (cap + ppw - 1) / ppw;
+
+    return (unsigned)(need < groups ? (need > 0 ? need : 1) : groups);
 }
 
 template <int N>
-hipError_t launch_emi_queue_n(const KParams &kp, const float2 *qt, const int *qpix,
-                              int qcount, float *coh, hipStream_t s)
+hipError_t launch_emi_queue_n(const KParams &kp, const EmiQueue &q, int pbase, int cap,
+                              float *coh, hipStream_t s)
 {
     typedef LinkShape<N> S;
-    const size_t lds = (size_t)S::EIG_BYTES + S::EMI_BYTES;
+    typedef RefineShape<N> R;
+    const size_t lq = (size_t)S::EIG_BYTES + (size_t)S::PPW * S::GJ_STRIDE;
+    const size_t lj = (size_t)S::EIG_BYTES + (size_t)S::PPW * S::JAC_STRIDE;
+    const size_t lr = (size_t)R::PPW * R::STRIDE;
+    hipError_t e;
 
-    if (qcount <= 0)
-        return hipSuccess;
-    if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&k_emi_queue<N>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess)
-            return e;
-    }
-    hipLaunchKernelGGL(k_emi_queue<N>, dim3((qcount + S::PPW - 1) / S::PPW),
-                       dim3(WAVE), lds, s, kp, qt, qpix, qcount, coh);
+    if ((e = lds_limit(reinterpret_cast<const void *>(&k_emi_queue<N>), lq)) != hipSuccess)
+        return e;
+    if ((e = lds_limit(reinterpret_cast<const void *>(&k_emi_jacobi<N>), lj)) != hipSuccess)
+        return e;
+    if ((e = lds_limit(reinterpret_cast<const void *>(&k_emi_refine<N>), lr)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_emi_queue<N>, dim3(queue_grid(cap, S::PPW, 4096)), dim3(WAVE), lq,
+                       s, kp, q, pbase, coh);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_emi_jacobi<N>, dim3(queue_grid(cap, S::PPW, 1024)), dim3(WAVE), lj,
+                       s, kp, q, pbase, coh);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_emi_refine<N>, dim3(queue_grid(cap, R::PPW, 8192)), dim3(WAVE), lr,
+                       s, kp, q, pbase, cap, coh);
     return hipGetLastError();
 }
 
 template <int N>
 hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
-                         const uint32_t *mask, const int *count, float *coh,
-                         float *pairs, int row0, int nrows, int precision,
-                         float2 *qt, int *qpix, int *qn, hipStream_t s)
+                         const short *ex, const uint32_t *mask, const int *count,
+                         float *coh, float *pairs, int row0, int nrows, int precision,
+                         const EmiQueue &q, hipStream_t s)
 {
-    const EmiQueue q = {qt, qpix, qn};
-
     typedef LinkShape<N> S;
     const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW));
     const int scratch = link_scratch_bytes<N>(kp, precision);
-    const size_t lds = (size_t)S::TL_BYTES + scratch +
-                       (kp.emi ? (qt ? S::CH_BYTES : S::EMI_BYTES) : 0);
+    const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::PPW * S::CH_STRIDE : 0);
 
+    if (kp.emi && !(q.t && q.z && q.how && q.pix && q.n && q.next))
+        return hipErrorInvalidValue;
 #define TCOH_LINK_LAUNCH(EMI)                                                   \
     do {                                                                        \
-        if (lds > 65536) {                                                      \
-            const hipError_t e = hipFuncSetAttribute(                           \
-                reinterpret_cast<const void *>(&k_link<N, EMI>),                \
-                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);          \
-            if (e != hipSuccess)                                                \
-                return e;                                                       \
-        }                                                                       \
+        const hipError_t e =                                                    \
+            lds_limit(reinterpret_cast<const void *>(&k_link<N, EMI>), lds);    \
+        if (e != hipSuccess)                                                    \
+            return e;                                                           \
         hipLaunchKernelGGL((k_link<N, EMI>), grid, dim3(WAVE), lds, s, kp, slc, \
-                           hl, mask, count, coh, pairs, row0, nrows, scratch,   \
-                           precision, q);                                       \
+                           hl, ex, mask, count, coh, pairs, row0, nrows,        \
+                           scratch, precision, q);                              \
     } while (0)
     if (kp.emi)
         TCOH_LINK_LAUNCH(1);
@@ -215,10 +354,9 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
 /* One translation unit per stack depth (the build compiles this file once
    for every N, in parallel). */
 template hipError_t launch_link_n<TCOH_EIG_N>(const KParams &, const float2 *,
-                                              const uint2 *, const uint32_t *,
-                                              const int *, float *, float *, int,
-                                              int, int, float2 *, int *, int *,
+                                              const uint2 *, const short *,
+                                              const uint32_t *, const int *, float *,
+                                              float *, int, int, int, const EmiQueue &,
                                               hipStream_t);
-template hipError_t launch_emi_queue_n<TCOH_EIG_N>(const KParams &, const float2 *,
-                                                   const int *, int, float *,
-                                                   hipStream_t);
+template hipError_t launch_emi_queue_n<TCOH_EIG_N>(const KParams &, const EmiQueue &, int,
+                                                   int, float *, hipStream_t);

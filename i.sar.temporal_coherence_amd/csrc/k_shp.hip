@@ -39,10 +39,7 @@ __device__ __forceinline__ float logf_rounded(float x)
 }
 
 /* ampT[k][pix] = amp[pix][k]; lstat[pix] = (mean, var) of log amplitudes
-   of valid pixels when the t-test is used; with absmax, the largest finite
-   amplitude of the strip is folded into *absmax (bits of a non-negative
-   float, one atomic per wavefront) for the split forms' scale. A block
-   stages its PREP_PX
+   of valid pixels when the t-test is used. A block stages its PREP_PX
    pixels' amplitudes (contiguous in amp) in LDS with coalesced reads, then
    writes the date planes; a row of the tile is padded to n + 1 words so
    that the per-pixel reads do not collide in LDS banks. */
@@ -52,8 +49,7 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
                                                  const float *__restrict__ amp,
                                                  float *__restrict__ ampT,
                                                  const unsigned char *__restrict__ valid,
-                                                 float2 *__restrict__ lstat,
-                                                 unsigned int *__restrict__ absmax)
+                                                 float2 *__restrict__ lstat)
 {
     __shared__ float tile[PREP_PX * (64 + 1)]; /* up to 64 dates */
     const int n = kp.n, n1 = n + 1;
@@ -64,25 +60,10 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
 
     /* t < 128 * 64: (t + 0.5) / n is at least 0.5 / n away from an
        integer, far more than the float product's error */
-    unsigned int mbits = 0u;
-
     for (int t = threadIdx.x; t < np * n; t += PREP_PX) {
         const int pp = (int)(((float)t + 0.5f) * inv_n);
-        const float v = src[t];
 
-        tile[pp * n1 + (t - pp * n)] = v;
-        if (isfinite(v))
-            mbits = max(mbits, __float_as_uint(fabsf(v)));
-    }
-    if (absmax) {
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1)
-            mbits = max(mbits, (unsigned int)__shfl_xor((int)mbits, o, 64));
-        /* read first: the stored maximum rises fast, so most wavefronts
-           skip the atomic and its contention on the one address */
-        if ((threadIdx.x & 63) == 0 &&
-            mbits > __hip_atomic_load(absmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(absmax, mbits);
+        tile[pp * n1 + (t - pp * n)] = src[t];
     }
     __syncthreads();
     if ((int)threadIdx.x >= np)
@@ -307,16 +288,58 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
     count[out] = looks;
 }
 
-__global__ void k_zero_u32(unsigned int *out) { *out = 0u; }
+/* The AD terms q(j, m) = (float)(2Nm - jN)^2 / (float)(j(2N - j)) of the
+   SHP walk, rounded twice as tcoh.cl:71-72 rounds them (the square, then
+   the correctly rounded quotient; no contraction in this file), for j <
+   2N and m <= N (row j = 0 is zero). */
+__global__ void k_adtab(int n, float *q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+
+    if (i >= 2 * n * (n + 1))
+        return;
+    const int j = i / (n + 1), m = i - j * (n + 1);
+
+    if (j == 0) {
+        q[i] = 0.0f;
+        return;
+    }
+    const float num = (float)(2 * n * m - j * n);
+    const float den = (float)(j * (2 * n - j));
+    const float sq = num * num;
+
+    q[i] = sq / den;
+}
+
+hipError_t launch_adtab(int n, float *q, hipStream_t s)
+{
+    const int total = 2 * n * (n + 1);
+
+    hipLaunchKernelGGL(k_adtab, dim3((total + 255) / 256), dim3(256), 0, s, n, q);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_words(uint32_t *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+
+    if (i < n)
+        p[i] = 0u;
+}
+
+hipError_t launch_zero_words(uint32_t *p, int n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_zero_words, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
-                       const unsigned char *valid, float2 *lstat,
-                       unsigned int *absmax, hipStream_t s)
+                       const unsigned char *valid, float2 *lstat, hipStream_t s)
 {
-    if (absmax)
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, absmax);
     hipLaunchKernelGGL(k_prep, dim3((kp.npad + PREP_PX - 1) / PREP_PX), dim3(PREP_PX),
-                       0, s, kp, amp, ampT, valid, lstat, absmax);
+                       0, s, kp, amp, ampT, valid, lstat);
     return hipGetLastError();
 }
 

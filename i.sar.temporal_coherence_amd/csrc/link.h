@@ -8,17 +8,21 @@
 
 #include "eig.h"
 
+/* |T|^-1 of EMI by INV: the Cholesky of k_link, the Gauss-Jordan inverse
+   of the first queue kernel, the Jacobi pseudo-inverse of the second. */
+enum { INV_CHOLESKY = 0, INV_GAUSS_JORDAN = 1, INV_JACOBI = 2 };
+
 /* gamma of the group's pixel, NaN when it is not estimated or (with the
-   bias correction) its dates are not linked. EMI with fallback false: a
-   pixel whose |T| fails the Cholesky pivot test is not linked here
-   (deferred set, the result is meaningless) and goes to the fallback
-   queue. */
-template <int N, int G, int EMI>
+   bias correction) its dates are not linked. EMI: a pixel whose |T| the
+   INV step cannot invert (Cholesky pivot test, condition bound) is not
+   linked here (deferred set, the result is meaningless) and goes to the
+   next queue. */
+template <int N, int G, int EMI, int INV>
 __device__ __forceinline__ float link_group(const KParams &kp,
                                             const float2 *tp, bool est,
                                             int lg, EigLds<N> &sh,
-                                            EmiLds<N> &em, bool fallback,
-                                            bool &deferred)
+                                            EmiLds<N> &em, bool &deferred,
+                                            float2 *zraw = nullptr)
 {
     deferred = false;
     constexpr int CPL = (N + G - 1) / G;
@@ -88,8 +92,15 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     }
     else {
         float inv[CPL][N];
+        bool ok = true;
 
-        if (!abs_inverse<N, G, CPL>(tp, est, em, lg, inv, fallback)) {
+        if constexpr (INV == INV_CHOLESKY)
+            ok = chol_inverse<N, G, CPL>(tp, est, em, lg, inv);
+        else if constexpr (INV == INV_GAUSS_JORDAN)
+            ok = gj_inverse<N, G, CPL>(tp, est, em, lg, inv);
+        else
+            jacobi_pinv<N, G, CPL>(tp, est, em, lg, inv);
+        if (!ok) {
             deferred = true;
 #pragma unroll
             for (int m = 0; m < CPL; m++)
@@ -113,6 +124,13 @@ __device__ __forceinline__ float link_group(const KParams &kp,
             }
         }
         hermitian_eigenvector<N, G, CPL, true>(A, sh, 0, lg, z);
+        /* the eigenvector itself, for the FP64 refinement (refine.h) */
+        if (zraw) {
+#pragma unroll
+            for (int m = 0; m < CPL; m++)
+                if (lg + m * G < N)
+                    zraw[lg + m * G] = z[m];
+        }
     }
 
     /* Phasors relative to date N/2 or to the last strongest component

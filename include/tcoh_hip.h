@@ -256,7 +256,12 @@ int tcoh_scene_from_metadata(const char *const *metas,
 
 /* Arithmetic of the covariance stage. PLAIN: FP32 FMA on the vector ALUs,
    as the reference kernel. MIXED_F16 / MIXED_BF16: hi/lo split 16-bit
-   operands, three products on the matrix cores with FP32 accumulation. */
+   operands, three products on the matrix cores with FP32 accumulation
+   (FP16: each pixel under its own power-of-two scale, each window rescaled
+   exactly to its largest candidate, so results depend on the window only).
+   The EMI estimator ignores the setting: its covariance always sums in
+   FP64 on the vector ALUs, since |T|^-1 o T amplifies the rounding of FP32
+   sums beyond 1e-4 in gamma where |T| is near singular. */
 enum tcoh_precision {
     TCOH_PLAIN = 0,
     TCOH_MIXED_F16 = 1,
@@ -322,6 +327,31 @@ void tcoh_run_region(struct ocl *o, int nrows, int ncols, const float *i,
 void tcoh_run_device(struct ocl *o, const void *slc, const void *amp,
                      const void *valid, int padded_rows, int rows, int cols,
                      void *coh, void *count, void *pairs, void *stream);
+
+/* tcoh_run_device plus the linked phase of every date: phases (device,
+   float[N][rows][cols], planar like pairs) receives arg(z_k) in radians of
+   the unit phasors that gamma is fitted with (tcoh.cl:696-724: relative to
+   date N/2 or the last strongest date), NaN where coh is NaN. Extension:
+   the reference keeps the phasors inside the kernel. */
+void tcoh_run_device_phases(struct ocl *o, const void *slc, const void *amp,
+                            const void *valid, int padded_rows, int rows,
+                            int cols, void *coh, void *count, void *pairs,
+                            void *phases, void *stream);
+
+/* ocl_run plus the linked phases (host buffer as above). Synchronous. */
+void tcoh_run_phases(struct ocl *o, const float *slc, const float *amp,
+                     const unsigned char *valid, int padded_rows, int rows,
+                     int cols, float *coh, int *count, float *pairs,
+                     float *phases);
+
+/* Graph mode (default off): tcoh_run_device and tcoh_run_device_phases
+   (and ocl_run) capture a strip's launches into a HIP graph on the first
+   call with given buffers, shape, stream and precision, and replay it with
+   one hipGraphLaunch on later calls with the same ones (a call with others
+   captures anew). Not while timing is on. The stream must be capturable
+   (not the legacy null stream); calls on one handle must not overlap.
+   Extension. */
+void tcoh_set_graph(struct ocl *o, int on);
 
 /* Device time of the last run's stages in milliseconds: [0] preparation
    (planar amplitudes, and the hi/lo split in the mixed forms), [1] SHP

@@ -29,12 +29,21 @@ reference's own oracle uses (reference.py:148-157).
 
 With form="f16" or "bf16", the covariance is the one the library's mixed
 forms define (no reference counterpart): each sample split as hi + lo
-16-bit values (round to nearest even; FP16 after a power-of-two scale that
-puts the largest amplitude in [2^13, 2^14), BF16 unscaled) and every
-product taken as hi.hi + hi.lo + lo.hi, so C_form = C(hi + lo) - lo lo^H.
-This is the arithmetic of the form, not of the reference: its distance
-from the exact C is the form's own error, which an ill-conditioned EMI
-estimate amplifies like any other input error.
+16-bit values (round to nearest even), every product taken as hi.hi +
+hi.lo + lo.hi, so C_form = C(hi + lo) - lo lo^H. FP16 first scales each
+pixel by the power of two that puts its largest component magnitude in
+[2^13, 2^14), then, per output pixel, multiplies every candidate's hi and
+lo by 2^-(its exponent below the window's largest accepted one) in FP16
+(exact unless the product is subnormal); BF16 takes no scale. This is the
+arithmetic of the form, not of the reference: its distance from the exact
+C is the form's own error.
+
+With adopt (bias correction only), the oracle takes the tested side's
+decision where the bias threshold is within rounding: for every entry
+whose |m2c| lies below band, |T_ij| is the tested side's (adopt: its
+|T_ij| for all pairs i < j, as a pairs=all run gives them; 0 where it
+zeroed the entry) and the phase is the FP64 one. Elsewhere everything is
+FP64.
 """
 
 import numpy as np
@@ -57,20 +66,15 @@ def mask_bits(mask, nbits):
     return b[:, :nbits].astype(bool)
 
 
-def covariance(slc, accepted, origin, offsets, lo=None):
+def covariance(slc, accepted, origin, offsets):
     """Hermitian C = sum_k s_k s_k^H over accepted candidates, complex128.
 
     slc: complex (npad, N); accepted: bool (P, K); origin: window origin
-    pixel index (P,); offsets (K,). lo (complex (npad, N), optional): the
-    lo parts of a split form, whose lo lo^H products the form leaves out.
+    pixel index (P,); offsets (K,).
     """
     idx = origin[:, None] + offsets[None, :]
     x = slc[idx] * accepted[..., None]  # (P, K, N)
-    c = np.einsum("pki,pkj->pij", x, x.conj(), optimize=True)
-    if lo is not None:
-        y = lo[idx] * accepted[..., None]
-        c = c - np.einsum("pki,pkj->pij", y, y.conj(), optimize=True)
-    return c
+    return np.einsum("pki,pkj->pij", x, x.conj(), optimize=True)
 
 
 def round_bf16(x):
@@ -80,29 +84,69 @@ def round_bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
+def split16(xs, form):
+    """(hi, lo) float64 of float32 values xs under the form's rounding."""
+    xs = np.asarray(xs, np.float32)
+    if form == "bf16":
+        hi = round_bf16(xs)
+        lo = round_bf16((xs - hi).astype(np.float32))
+    else:
+        hi = xs.astype(np.float16).astype(np.float32)
+        lo = (xs - hi).astype(np.float32).astype(np.float16).astype(np.float32)
+    return hi.astype(np.float64), lo.astype(np.float64)
+
+
+def split_exponents(x):
+    """Per pixel (leading axis) the FP16 form's scale exponent: that of the
+    largest finite component magnitude (frexp), at least -113; None for a
+    pixel without a non-zero one."""
+    x = np.asarray(x, np.float32).reshape(x.shape[0], -1)
+    m = np.where(np.isfinite(x), np.abs(x), 0).max(axis=1)
+    e = np.maximum(np.frexp(m)[1], -113)
+    return np.where(m > 0, e, -32768)
+
+
 def split_parts(slc32, form):
-    """(hi + lo, lo) of float32 samples under a mixed form's split, as
-    float64 arrays of the unscaled values."""
+    """Per-pixel split of float32 samples (P, N, 2): (hi, lo) float64 of
+    the scaled values and the scale exponents (FP16: each pixel times
+    2^(14 - e)); BF16 unscaled, exponents 0."""
     x = np.asarray(slc32, np.float32)
     if form == "bf16":
-        scale = 1.0
-        rnd = round_bf16
-    else:
-        amp = np.sqrt(x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]).astype(np.float32)
-        fin = amp[np.isfinite(amp)]
-        mx = float(fin.max()) if fin.size else 0.0
-        scale = 2.0 ** (14 - np.frexp(np.float32(mx))[1]) if mx > 0 else 1.0
-        def rnd(v):
-            return np.asarray(v, np.float32).astype(np.float16).astype(np.float32)
-    xs = (x * np.float32(scale)).astype(np.float32)
-    hi = rnd(xs)
-    lo = rnd((xs - hi).astype(np.float32))
-    return ((hi.astype(np.float64) + lo.astype(np.float64)) / scale,
-            lo.astype(np.float64) / scale)
+        hi, lo = split16(x, form)
+        return hi, lo, np.zeros(x.shape[0], np.int64)
+    e = split_exponents(x)
+    sc = np.ldexp(np.float32(1), np.where(e == -32768, 0, 14 - e)).astype(np.float32)
+    hi, lo = split16(x * sc.reshape((-1,) + (1,) * (x.ndim - 1)), form)
+    return hi, lo, e
 
 
-def coherence(c, looks, bias):
-    """tcoh.cl:610-636 in double: normalised, optionally debiased T."""
+def form_covariance(parts, accepted, origin, offsets, form):
+    """C of a mixed form (P, N, N) complex128 from split_parts output: per
+    output pixel, FP16 rescales each accepted candidate's hi and lo by
+    2^-(window max exponent - its exponent) with FP16 rounding, then C =
+    sum (hi + lo)(hi + lo)^H - lo lo^H (in units of the window's common
+    scale, to which T is invariant)."""
+    hi, lo, e = parts
+    idx = origin[:, None] + offsets[None, :]
+    h, l = hi[idx], lo[idx]  # (P, K, N, 2)
+    if form != "bf16":
+        ek = np.where(accepted, e[idx], np.iinfo(np.int64).min)
+        emax = ek.max(axis=1, keepdims=True)
+        sh = np.minimum(emax - np.where(accepted, e[idx], emax), 31)
+        f = np.where(sh <= 24, np.ldexp(1.0, -sh), 0.0)[..., None, None]
+        h = (h * f).astype(np.float16).astype(np.float64)
+        l = (l * f).astype(np.float16).astype(np.float64)
+    acc = accepted[..., None]
+    full = (h[..., 0] + 1j * h[..., 1] + l[..., 0] + 1j * l[..., 1]) * acc
+    low = (l[..., 0] + 1j * l[..., 1]) * acc
+    return (np.einsum("pki,pkj->pij", full, full.conj(), optimize=True) -
+            np.einsum("pki,pkj->pij", low, low.conj(), optimize=True))
+
+
+def coherence(c, looks, bias, band=None, adopt=None):
+    """tcoh.cl:610-636 in double: normalised, optionally debiased T. With
+    band and adopt (P, N(N-1)/2): the tested side's |T_ij| where |m2c| <
+    band (see the module notes)."""
     n = c.shape[-1]
     d = np.sqrt(np.maximum(np.real(np.einsum("pii->pi", c)), 0.0))
     den = d[:, :, None] * d[:, None, :]
@@ -115,6 +159,13 @@ def coherence(c, looks, bias):
             f = np.where((m2 > 0) & (m2c > 0) & (lk > 1),
                          np.sqrt(np.where(m2 > 0, m2c / np.where(m2 > 0, m2, 1), 0)),
                          np.where(lk > 1, 0.0, 1.0))
+            if adopt is not None:
+                iu, ju = np.triu_indices(n, 1)
+                want = np.zeros_like(m2)
+                want[:, iu, ju] = adopt
+                near = (np.abs(m2c) < band) & np.triu(np.ones((n, n), bool), 1)[None]
+                f = np.where(near, want / np.sqrt(np.where(m2 > 0, m2, 1.0)), f)
+                f = np.where(near & ~(m2 > 0), 0.0, f)
             t = t * f
     iu = np.arange(n)
     t[:, iu, iu] = 1.0
@@ -222,8 +273,10 @@ def gamma(t, z):
     return np.abs(s) / (0.5 * n * (n - 1))
 
 
-def estimate(t, emi, loading=0.0):
-    """Linked phases of each T (P, N, N): unit phasors (P, N)."""
+def estimate(t, emi, loading=0.0, mag_out=None):
+    """Linked phases of each T (P, N, N): unit phasors (P, N). mag_out (P, N)
+    optional: receives |u_k| / max |u| of the eigenvector (the phasor
+    floor's test, tcoh.cl:696-724)."""
     if not emi:
         _w, v = np.linalg.eigh(t)
         u = v[:, :, -1]
@@ -231,11 +284,15 @@ def estimate(t, emi, loading=0.0):
         inv, _ok = abs_inverse(t, loading)
         _w, v = np.linalg.eigh(inv * t)
         u = v[:, :, 0]
+    if mag_out is not None:
+        m = np.abs(u)
+        mag_out[...] = m / m.max(axis=1, keepdims=True)
     return phasors(u)
 
 
 def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
-        chunk=4096, emi_loading=0.0, band=None, form=None):
+        chunk=4096, emi_loading=0.0, band=None, form=None, adopt=None, z_out=None,
+        zmag_out=None):
     """FP64 coherence and pair coherences of the strip's output pixels.
 
     slc: float32 (npad, N, 2) or complex; count: int (rows*cols,) from the
@@ -246,15 +303,17 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
     where not estimated, like the kernel. With band (bias correction only)
     also returns near (len(pixels),): the number of pairs i < j with
     |m2c| < band, and m2c (len(pixels), npairs) of the output pairs (NaN
-    where not estimated).
+    where not estimated). adopt (len(pixels), N(N-1)/2): the tested side's
+    |T_ij| of every pair i < j, taken where |m2c| < band (module notes).
+    z_out (len(pixels), N) complex, optional: receives the unit phasors of
+    the linked pixels (NaN elsewhere); zmag_out (len(pixels), N) their
+    eigenvector magnitudes relative to the largest.
     """
     n = settings["ndates"]
     wa, wr = settings["win_az"], settings["win_rg"]
-    lo_parts = None
+    parts = None
     if form in ("f16", "bf16"):
-        full, low = split_parts(np.asarray(slc, np.float32).reshape(-1, n, 2), form)
-        slc = full[..., 0] + 1j * full[..., 1]
-        lo_parts = low[..., 0] + 1j * low[..., 1]
+        parts = split_parts(np.asarray(slc, np.float32).reshape(-1, n, 2), form)
     elif slc.dtype != np.complex128:
         slc = np.asarray(slc, np.float32).reshape(-1, n, 2)
         slc = slc[..., 0].astype(np.float64) + 1j * slc[..., 1].astype(np.float64)
@@ -272,6 +331,13 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
     pairs = np.full((len(pixels), npairs), np.nan)
     near = np.zeros(len(pixels), np.int64)
     pm2c = np.full((len(pixels), npairs), np.nan)
+    if z_out is not None:
+        z_out[...] = np.nan
+    if zmag_out is not None:
+        zmag_out[...] = np.nan
+    use_adopt = adopt is not None and settings["bias"]
+    if use_adopt and band is None:
+        raise ValueError("adopt needs band")
     for lo in range(0, len(pixels), chunk):
         sel = pixels[lo:lo + chunk]
         cnt = count[sel]
@@ -282,9 +348,13 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
         y, x = p // cols, p % cols
         origin = y * padded_cols + x
         acc = mask_bits(mask[p], wa * wr)
-        c = covariance(slc, acc, origin, offs, lo_parts)
-        t = coherence(c, cnt[est], settings["bias"])
+        if parts is not None:
+            c = form_covariance(parts, acc, origin, offs, form)
+        else:
+            c = covariance(slc, acc, origin, offs)
         out = np.flatnonzero(est) + lo
+        t = coherence(c, cnt[est], settings["bias"], band,
+                      adopt[out] if use_adopt else None)
         if band is not None and settings["bias"]:
             m2c = bias_m2c(c, cnt[est])
             up = np.triu(np.ones((n, n), bool), 1)
@@ -295,8 +365,13 @@ def run(slc, count, mask, settings, padded_cols, rows, cols, pixels=None,
             pairs[out] = np.abs(t[:, pi, pj])
         ok = connected(t) if settings["bias"] else np.ones(len(p), bool)
         if ok.any():
-            z = estimate(t[ok], settings["emi"], emi_loading)
+            mg = np.empty((int(ok.sum()), n)) if zmag_out is not None else None
+            z = estimate(t[ok], settings["emi"], emi_loading, mg)
             coh[out[ok]] = gamma(t[ok], z)
+            if z_out is not None:
+                z_out[out[ok]] = z
+            if zmag_out is not None:
+                zmag_out[out[ok]] = mg
     if band is not None:
         return coh, pairs, near, pm2c
     return coh, pairs

@@ -242,3 +242,69 @@ def test_raw_region_writes_the_module_maps(tcoh, tmp_path):
         assert np.array_equal(g[~np.isnan(g)], pairs[p][~np.isnan(pairs[p])])
     assert est == int(np.sum(~nan)) and est > 100
     assert mean == pytest.approx(float(np.mean(coh[~nan], dtype=np.float64)), rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["plain", "f16"])
+def test_depth60_with_geometric_phase(tcoh, cref, fp64, precision):
+    """The full-geometry workload's path on a crop: N=60 dates carrying
+    exp(-i phi_geo) of a 0-500 m DEM (phi_geo from the library's FP64 grid
+    on circular orbits), removed again by the device strip preparation
+    (k_strip at 64 date slots) at the DEM heights, then estimated by the
+    N=60 kernels (KS, EVD, bias, consecutive pairs). Against the oracle's
+    strip (read_strip with geometry_phases at the same heights), its SHP
+    counts (bit for bit) and the FP64 gamma and pairs (1e-4)."""
+    from test_gpu_parity import M2C_BAND
+    D = tcoh.api.DeviceBuffer
+    n, rows, cols = 60, 34, 30
+    window = (21, 7)
+    ha, hr = window[0] // 2, window[1] // 2
+    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, ref=0, seed=7)
+    grid, hs = tcoh.api.geometry_grid(orbits, scene, ref, 0.0, 500.0)
+    g = tcoh.api.make_geometry(grid, hs)
+    yy, xx = np.mgrid[0:rows, 0:cols]
+    dem = 250.0 + 240.0 * np.sin(yy / 7.0) * np.cos(xx / 5.0)
+    dem[5, 6] = np.nan
+    phi = cref.pixel_phases(grid, hs, rows, cols, elevation=dem)
+    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=3,
+                                            gamma0=(0.3, 0.9))
+    obs = tcoh.synth.scene_stack(n, rows, cols, reg, params, seed=3) * np.exp(-1j * phi)
+    i = np.ascontiguousarray(obs.real, np.float32)
+    q = np.ascontiguousarray(obs.imag, np.float32)
+    prow, pc = rows + 2 * ha, cols + 2 * hr
+    want = cref.prepare_strip(i, q, -ha, prow, hr, phase=phi,
+                              elev_null=np.isnan(dem).astype(np.uint8))
+    s = tcoh.make_settings(n, window=window, min_shp=20, bias=True, pairs="consecutive")
+    s_all = tcoh.make_settings(n, window=window, min_shp=20, bias=True, pairs="all")
+    di, dq, dh = D.from_array(i), D.from_array(q), D.from_array(dem)
+    slc, amp, valid = D(prow * pc * n * 8), D(prow * pc * n * 4), D(prow * pc)
+    npx = rows * cols
+    coh, count, pairs = D(npx * 4), D(npx * 4), D((n - 1) * npx * 4)
+    pall = D(n * (n - 1) // 2 * npx * 4)
+    with tcoh.Backend(s, pc, platform=0, device=0, precision=precision) as be, \
+            tcoh.Backend(s_all, pc, platform=0, device=0, precision=precision) as ball:
+        be.prepare_strip_device(di.ptr, dq.ptr, rows, cols, -ha, prow, hr, slc.ptr,
+                                amp.ptr, valid.ptr, geometry=g, heights_ptr=dh.ptr)
+        be.run_device(slc.ptr, amp.ptr, valid.ptr, prow, rows, cols, coh.ptr, count.ptr,
+                      pairs.ptr)
+        got_coh = coh.to_array(np.float32)
+        got_count = count.to_array(np.int32)
+        got_pairs = pairs.to_array(np.float32, (n - 1, npx))
+        ball.run_device(slc.ptr, amp.ptr, valid.ptr, prow, rows, cols, coh.ptr, count.ptr,
+                        pall.ptr)
+        all_pairs = pall.to_array(np.float32, (n * (n - 1) // 2, npx))
+        assert np.array_equal(coh.to_array(np.float32).view(np.uint32),
+                              got_coh.view(np.uint32))
+        got_valid = valid.to_array(np.uint8)
+    assert np.array_equal(got_valid, want[2])
+    so = cref.settings(n, window=window, min_shp=20, bias=True, pairs="consecutive")
+    _c, ocount, _p, mask, _f = cref.run(so, pc, want[0], want[1], want[2], prow, rows, cols)
+    assert np.array_equal(got_count, ocount)
+    assert (ocount >= so.min_looks).sum() > 500
+    c64, p64, _near, _m = fp64.run(want[0], ocount, mask, cref.settings_dict(so), pc, rows,
+                                   cols, band=M2C_BAND[precision],
+                                   adopt=all_pairs.T.astype(np.float64))
+    assert np.array_equal(np.isnan(got_coh), np.isnan(c64))
+    err = np.nanmax(np.abs(got_coh - c64))
+    assert err <= 1e-4, err
+    assert np.nanmax(np.abs(got_pairs.T - p64)) <= 1e-4
+    assert np.isnan(got_coh.reshape(rows, cols)[5, 6])

@@ -350,30 +350,126 @@ def test_emi_fallback_scene_reaches_the_jacobi_path(cref, fp64, tcoh):
 
 def test_fp64_split_forms(fp64):
     """The mixed forms' covariance in the FP64 oracle: round to nearest even
-    (BF16 ties to even by bits, FP16 by numpy's IEEE conversion), hi + lo
-    within the form's precision of the sample, and C_form = C(hi + lo) -
-    lo lo^H equal to the sum of the hi.hi, hi.lo and lo.hi products of the
-    real Gram matrices (C_re = Re Re^T + Im Im^T, C_im = Im Re^T - Re Im^T)."""
+    (BF16 ties to even by bits, FP16 by numpy's IEEE conversion); FP16
+    scales each pixel so that its largest component lies in [2^13, 2^14);
+    hi + lo within the form's precision of the (scaled) sample; and the
+    form's C equal to the sum of the hi.hi, hi.lo and lo.hi products of the
+    real Gram matrices (C_re = Re Re^T + Im Im^T, C_im = Im Re^T - Re Im^T)
+    of the candidates rescaled to the window's largest exponent."""
     # 1 + 2^-8 is halfway between BF16 neighbours 1 and 1 + 2^-7: to even
     x = np.array([1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -2.5, 0.0], np.float32)
     assert list(fp64.round_bf16(x)) == [1.0, 1 + 2.0 ** -6, -2.5, 0.0]
     rng = np.random.default_rng(3)
     n, k = 7, 40
     xs = (rng.normal(size=(k, n, 2)) * 300).astype(np.float32)
+    # one pixel 60 dB (x1000) brighter, one 2^-20 dimmer: other exponents
+    xs[5] *= 1000.0
+    xs[9] *= 2.0 ** -20
     for form, prec in (("f16", 2.0 ** -21), ("bf16", 2.0 ** -15)):
-        full, lo = fp64.split_parts(xs, form)
-        assert np.all(np.abs(full - xs) <= prec * np.abs(xs) + 1e-30)
-        hi = full - lo
-        s_full = full[..., 0] + 1j * full[..., 1]
-        s_lo = lo[..., 0] + 1j * lo[..., 1]
+        hi, lo, e = fp64.split_parts(xs, form)
+        scale = np.ldexp(1.0, 14 - e) if form == "f16" else np.ones(k)
+        xsc = xs * scale[:, None, None]
+        if form == "f16":
+            top = np.abs(hi).reshape(k, -1).max(axis=1)
+            assert np.all((top >= 2.0 ** 13) & (top <= 2.0 ** 14))
+        assert np.all(np.abs(hi + lo - xsc) <= prec * np.abs(xsc) + 1e-30)
         acc = np.ones((1, k), bool)
-        c = fp64.covariance(s_full, acc, np.array([0]), np.arange(k), s_lo)[0]
-        rh, ih, rl, il = hi[..., 0], hi[..., 1], lo[..., 0], lo[..., 1]
+        c = fp64.form_covariance((hi, lo, e), acc, np.array([0]), np.arange(k), form)[0]
+        if form == "f16":
+            # v_pk_mul_f16 by 2^-shift: FP16 results
+            f = np.ldexp(1.0, -(e.max() - e))[:, None, None]
+            h16 = (hi * f).astype(np.float16).astype(np.float64)
+            l16 = (lo * f).astype(np.float16).astype(np.float64)
+            rh, ih, rl, il = h16[..., 0], h16[..., 1], l16[..., 0], l16[..., 1]
+        else:
+            rh, ih, rl, il = hi[..., 0], hi[..., 1], lo[..., 0], lo[..., 1]
 
         def gram(a_h, a_l, b_h, b_l):
             return a_h.T @ b_h + a_h.T @ b_l + a_l.T @ b_h
 
         cre = gram(rh, rl, rh, rl) + gram(ih, il, ih, il)
         cim = gram(ih, il, rh, rl) - gram(rh, rl, ih, il)
-        assert np.allclose(c.real, cre, rtol=1e-13, atol=1e-9)
-        assert np.allclose(c.imag, cim, rtol=1e-13, atol=1e-9)
+        tol = 1e-9 * np.abs(cre).max()
+        assert np.allclose(c.real, cre, rtol=1e-12, atol=tol)
+        assert np.allclose(c.imag, cim, rtol=1e-12, atol=tol)
+        # and the form's T close to the exact one despite the 60 dB spread
+        x64 = xs[..., 0].astype(np.float64) + 1j * xs[..., 1]
+        c64 = fp64.covariance(x64, acc, np.array([0]), np.arange(k))[0]
+        t64 = fp64.coherence(c64[None], np.array([k]), False)[0]
+        tf = fp64.coherence(c[None], np.array([k]), False)[0]
+        assert np.max(np.abs(tf - t64)) < (1e-6 if form == "f16" else 1e-4)
+
+
+def test_fp16_split_depends_on_the_window_only(fp64):
+    """Under the FP16 form a window's covariance does not depend on samples
+    outside it: a pixel elsewhere of any finite magnitude (here 1e30 and
+    1e-30) leaves its T bit for bit unchanged, as in the reference's FP32
+    arithmetic."""
+    rng = np.random.default_rng(4)
+    xs = (rng.normal(size=(30, 6, 2)) * 50).astype(np.float32)
+    acc = np.ones((1, 20), bool)
+    base = fp64.form_covariance(fp64.split_parts(xs, "f16"), acc, np.array([0]),
+                                np.arange(20), "f16")
+    for v in (1e30, 1e-30):
+        ys = xs.copy()
+        ys[25] = v
+        other = fp64.form_covariance(fp64.split_parts(ys, "f16"), acc, np.array([0]),
+                                     np.arange(20), "f16")
+        assert np.array_equal(base, other)
+
+
+def test_prepare_strip_against_numpy(cref):
+    """oracle_prepare_strip (main.c:52-127) against a numpy statement of the
+    same contract: rows outside the region and the halo invalid; a pixel
+    valid when every date is non-null and not (0, 0) and its height is not
+    null; amplitudes float32 sqrt(I^2 + Q^2), each operation rounded to
+    float32, sorted increasingly, taken before the rotation; samples times
+    exp(+i phase) in double, then rounded to float32; invalid pixels all
+    zero."""
+    rng = np.random.default_rng(12)
+    n, nrows, ncols, halo = 6, 11, 13, 3
+    i = rng.normal(0, 50, (n, nrows, ncols)).astype(np.float32)
+    q = rng.normal(0, 50, (n, nrows, ncols)).astype(np.float32)
+    i[2, 4, 5] = np.nan
+    i[1, 6, 7] = q[1, 6, 7] = 0.0
+    q[3, 2, 2] = np.nan
+    phase = rng.uniform(-40.0, 40.0, (n, nrows, ncols))
+    en = np.zeros((nrows, ncols), np.uint8)
+    en[8, 1] = 1
+    row0, prow = -2, 15
+    pc = ncols + 2 * halo
+    for ph, el in ((None, None), (phase, en)):
+        slc, amp, valid = cref.prepare_strip(i, q, row0, prow, halo, ph, el)
+        slc = slc.reshape(prow, pc, n, 2)
+        amp = amp.reshape(prow, pc, n)
+        valid = valid.reshape(prow, pc)
+        want_v = np.zeros((prow, pc), np.uint8)
+        want_s = np.zeros((prow, pc, n, 2), np.float32)
+        want_a = np.zeros((prow, pc, n), np.float32)
+        for r in range(prow):
+            row = row0 + r
+            if not 0 <= row < nrows:
+                continue
+            vi, vq = i[:, row, :], q[:, row, :]  # (n, ncols)
+            ok = ~(np.isnan(vi) | np.isnan(vq) | ((vi == 0) & (vq == 0)))
+            ok = ok.all(axis=0)
+            if el is not None:
+                ok &= el[row] == 0
+            a = np.sqrt((vi * vi + vq * vq).astype(np.float32)).astype(np.float32)
+            a = np.sort(a, axis=0)
+            re, im = vi.astype(np.float64), vq.astype(np.float64)
+            if ph is not None:
+                cs, sn = np.cos(ph[:, row, :]), np.sin(ph[:, row, :])
+                re, im = re * cs - im * sn, re * sn + im * cs
+            for c in np.flatnonzero(ok):
+                want_v[r, halo + c] = 1
+                want_a[r, halo + c] = a[:, c]
+                want_s[r, halo + c, :, 0] = re[:, c].astype(np.float32)
+                want_s[r, halo + c, :, 1] = im[:, c].astype(np.float32)
+        assert np.array_equal(valid, want_v)
+        assert valid.sum() > 100 and valid.sum() < nrows * ncols
+        assert np.array_equal(amp, want_a)
+        # numpy's and the C library's double sin/cos may differ in the last
+        # bit; after rounding to float32 that shows at most as one float ulp
+        d = np.abs(slc.view(np.int32).astype(np.int64) - want_s.view(np.int32))
+        assert d.max() <= 1 and np.mean(d == 0) > 0.999
