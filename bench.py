@@ -29,6 +29,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))  # the synthetic scene generators
+
+import synth  # noqa: E402
 
 METRIC = "Mpixel/s at N=30, 21x7 KS+EVD: mixed-prec vs plain FP32 GPU, |dγ|<=1e-4"
 METRIC_EMI = "Mpixel/s at N=30, 21x7 AD+EMI, all pairs: mixed-prec vs plain FP32 GPU"
@@ -89,9 +92,9 @@ def config5_inputs(tcoh, torch, s, n, rows, cols, hr):
     rotation (the grid negated). Returns I/Q planes on the GPU, the heights
     on the GPU and the geometry struct the timed preparation removes."""
     import torch.nn.functional as F
-    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
+    reg, params = synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
                                             gamma0=(0.05, 0.95))
-    stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
+    stack = synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
     i = stack.real.contiguous()
     q = stack.imag.contiguous()
     del stack
@@ -102,7 +105,7 @@ def config5_inputs(tcoh, torch, s, n, rows, cols, hr):
     dem = F.interpolate(coarse, size=(rows, cols), mode="bicubic", align_corners=True)[0, 0]
     dem = (dem - dem.min()) / (dem.max() - dem.min()) * 500.0
     heights = dem.contiguous()
-    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, ref=0, seed=5)
+    orbits, scene, ref = synth.geometry_case(n=n, rows=rows, cols=cols, ref=0, seed=5)
     grid, hs = tcoh.api.geometry_grid(orbits, scene, ref, 0.0, 500.0)
     g = tcoh.api.make_geometry(grid, hs)
     fwd = tcoh.api.make_geometry(-grid, hs)
@@ -397,9 +400,9 @@ def main():
     if c5:
         di, dq, dheights, geo = config5_inputs(tcoh, torch, s, n, rows, cols, hr)
     else:
-        reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
+        reg, params = synth.config_voronoi(n=n, rows=rows, cols=cols, seed=0,
                                                 gamma0=gamma0)
-        stack = tcoh.synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
+        stack = synth.scene_stack_torch(n, rows, cols, reg, params, seed=0)
         di, dq = stack.real.contiguous(), stack.imag.contiguous()
         del stack
     slc, amp, valid, prep_ms = prepare_on_device(tcoh, torch, s, di, dq, ha, hr,

@@ -2,8 +2,7 @@
 
 The product is libtcoh_hip.so (HIP kernels behind the C ABI of
 include/tcoh_hip.h, a drop-in for the reference's ocl.c). This package holds
-its sources (csrc/, Makefile), the Python binding (api) and synthetic
-scene generators (synth).
+its sources (csrc/, Makefile) and the Python binding (api).
 
 The directory name is not a Python identifier; load it by path, e.g.
 
@@ -11,6 +10,6 @@ The directory name is not a Python identifier; load it by path, e.g.
         "tcoh_amd", path + "/__init__.py", submodule_search_locations=[path])
 """
 
-from . import api, synth  # noqa: F401
+from . import api  # noqa: F401
 from .api import (Backend, make_settings, pair_list, prepare_strip,  # noqa: F401
                   temporal_coherence)

@@ -208,7 +208,7 @@ struct ocl {
     DevBuf ampT, lstat, mask, adtab, hl, ex;
     /* EMI band state (EmiQueue): T, eigenvectors and inversion kind of the
        band's pixels, the two queues and their counts */
-    DevBuf qt, qz, qhow, qpix, qn, qnext;
+    DevBuf qt, qz, qref, qpix, qn, qnext;
     /* the phase grid of tcoh_prepare_strip_device / tcoh_run_region */
     DevBuf grid;
     /* tcoh_run_region: double-buffered date planes and outputs on the
@@ -389,7 +389,7 @@ struct StripScratch {
 
 /* Sizes of the strip's scratch buffers (bytes; 0: not used), in the order
    of StripScratch. */
-enum { SC_AMPT, SC_LSTAT, SC_MASK, SC_ADTAB, SC_HL, SC_EX, SC_QT, SC_QZ, SC_QHOW, SC_QPIX,
+enum { SC_AMPT, SC_LSTAT, SC_MASK, SC_ADTAB, SC_HL, SC_EX, SC_QT, SC_QZ, SC_QREF, SC_QPIX,
        SC_QNEXT, SC_QN, SC_COUNT };
 struct ScratchPlan {
     size_t bytes[SC_COUNT];
@@ -482,10 +482,10 @@ static ScratchPlan scratch_plan(const KParams &kp, int precision)
 
         pl.bytes[SC_QT] = cap * nt * sizeof(float2);
         pl.bytes[SC_QZ] = cap * n * sizeof(float2);
-        pl.bytes[SC_QHOW] = cap;
+        pl.bytes[SC_QREF] = cap * sizeof(int);
         pl.bytes[SC_QPIX] = cap * sizeof(int);
         pl.bytes[SC_QNEXT] = cap * sizeof(int);
-        pl.bytes[SC_QN] = 2 * sizeof(int);
+        pl.bytes[SC_QN] = 3 * sizeof(int);
     }
     return pl;
 }
@@ -500,7 +500,7 @@ static StripScratch scratch_pointers(void *const p[SC_COUNT])
     sc.adtab = (float *)p[SC_ADTAB];
     sc.hl = (uint32_t *)p[SC_HL];
     sc.ex = (short *)p[SC_EX];
-    sc.q = {(float2 *)p[SC_QT], (float2 *)p[SC_QZ], (unsigned char *)p[SC_QHOW],
+    sc.q = {(float2 *)p[SC_QT], (float2 *)p[SC_QZ], (int *)p[SC_QREF],
             (int *)p[SC_QPIX], (int *)p[SC_QN], (int *)p[SC_QNEXT]};
     return sc;
 }
@@ -509,7 +509,7 @@ static StripScratch scratch_pointers(void *const p[SC_COUNT])
 static StripScratch handle_scratch(struct ocl *o, const ScratchPlan &pl)
 {
     DevBuf *bufs[SC_COUNT] = {&o->ampT, &o->lstat, &o->mask, &o->adtab, &o->hl, &o->ex,
-                              &o->qt,   &o->qz,    &o->qhow, &o->qpix,  &o->qnext, &o->qn};
+                              &o->qt,   &o->qz,    &o->qref, &o->qpix,  &o->qnext, &o->qn};
     void *p[SC_COUNT];
 
     for (int k = 0; k < SC_COUNT; k++) {
@@ -571,7 +571,7 @@ static void enqueue_kernels(struct ocl *o, const KParams &kp, const ScratchPlan 
 
         mark(2, true);
         if (kp.emi)
-            check(launch_zero_words((uint32_t *)sc.q.n, 2, st), "queue count kernel");
+            check(launch_zero_words((uint32_t *)sc.q.n, 3, st), "queue count kernel");
         check(launch_link(kp, (const float2 *)slc, (const uint2 *)sc.hl, sc.ex, sc.mask,
                           (const int *)count, (float *)coh, (float *)pairs, r0, nr,
                           o->precision, sc.q, st),
@@ -1067,7 +1067,7 @@ extern "C" void ocl_close(struct ocl *o)
         DevBuf *bufs[] = {&o->slc,  &o->amp,   &o->valid, &o->coh,
                           &o->count, &o->pairs, &o->phase, &o->ampT,  &o->lstat,
                           &o->mask, &o->adtab, &o->hl,
-                          &o->ex, &o->qt, &o->qz, &o->qhow, &o->qpix, &o->qn, &o->qnext,
+                          &o->ex, &o->qt, &o->qz, &o->qref, &o->qpix, &o->qn, &o->qnext,
                           &o->grid, &o->rin[0], &o->rin[1],
                           &o->rhgt[0], &o->rhgt[1], &o->rout[0], &o->rout[1],
                           &o->rslc, &o->ramp, &o->rvalid};

@@ -418,18 +418,20 @@ hipError_t launch_strip(const StripArgs &sa, hipStream_t s);
    estimated pixel (NT float2 each at t) and appends the pixels whose |T|
    fails the Cholesky test to queue 1 (pix, n[0] entries); the first queue
    kernel inverts theirs by Gauss-Jordan and appends those beyond its
-   condition bound to queue 2 (next, n[1] entries) for the Jacobi. Every
-   linking kernel leaves the FP32 eigenvector (N float2 at z) and how the
-   pixel was inverted (how: 0 not refined, 1 Cholesky, 2 Gauss-Jordan) for
-   the FP64 refinement kernel. n is cleared before each band. */
+   condition bound to queue 2 (next, n[1] entries) for the Jacobi. The
+   linking kernels append the pixels whose FP32 eigenvector needs the FP64
+   refinement to queue 3 (ref, n[2] entries: bp | kind << 30, kind 1 for a
+   Cholesky inverse, 2 for a Gauss-Jordan one), with their eigenvector (N
+   float2 at z). n is cleared before each band. */
 struct EmiQueue {
     float2 *t;
     float2 *z;
-    unsigned char *how;
+    int *ref;
     int *pix;
     int *n;
     int *next;
 };
+#define REFINE_KIND_SHIFT 30
 /* Fused covariance + linking of output rows row0 .. row0+nrows-1
    (precision: 0 plain, 1 split FP16, 2 split BF16; EMI always plain with
    FP64 sums), one instantiation per stack depth N. EMI appends the pixels

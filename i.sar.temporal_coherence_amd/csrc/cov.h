@@ -237,12 +237,11 @@ __device__ __forceinline__ Opnd make_opnd(const uint2 (&v)[8])
 
 /* Candidate lists of the mixed form: the strip pixel index of every
    accepted candidate, padded to a multiple of 32 with the zero pixel
-   (index npad); in the FP16 form the top bits hold the candidate's
-   exponent below the window's largest (strip pixels stay below 2^27,
-   since npad * NB <= INT32_MAX with NB >= 16). */
-#define LIST_SHIFT_BITS 27
+   (index npad); in the FP16 form a parallel list holds each candidate's
+   rescaling factor 2^-(its exponent below the window's largest), as a pair
+   of FP16 values. */
 #define LIST_SHIFT_MAX 31
-#define LIST_INDEX_MASK 0x7FFFFFFu
+#define F16_ONE_PAIR 0x3C003C00u
 
 /* FP16 bits of 2^-s, 0 <= s <= 31 (subnormal from s = 15, zero above
    24), in both halves of a word. */
@@ -268,12 +267,13 @@ __host__ __device__ __forceinline__ int mfma_list_stride(int kw)
     return (kw + 31) & ~31;
 }
 
-/* LDS bytes of the mixed form for ppw pixels: lists, the diagonal of C,
-   the pixels' mask words, and a per-pixel table (chunks, looks). */
+/* LDS bytes of the mixed form for ppw pixels: lists and their factors,
+   the diagonal of C, the pixels' mask words, and a per-pixel table
+   (chunks, looks). */
 __host__ __device__ __forceinline__ int mfma_scratch_bytes(int ppw, int kw,
                                                            int words, int nb)
 {
-    return ppw * mfma_list_stride(kw) * 4 + nb * 4 + ppw * words * 4 + ppw * 8;
+    return 2 * ppw * mfma_list_stride(kw) * 4 + nb * 4 + ppw * words * 4 + ppw * 8;
 }
 
 /* Mixed form over the group's pixels x0 .. x0+PPW-1 of output row y, the
@@ -303,7 +303,8 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     const int kw = kp.wa * kp.wr, nw = kp.words;
     const int lstride = mfma_list_stride(kw);
     int *list = reinterpret_cast<int *>(scratch);
-    float *diag = reinterpret_cast<float *>(scratch + PPW * lstride * 4);
+    uint32_t *fac = reinterpret_cast<uint32_t *>(scratch + PPW * lstride * 4);
+    float *diag = reinterpret_cast<float *>(scratch + 2 * PPW * lstride * 4);
     uint32_t *mw = reinterpret_cast<uint32_t *>(diag + NB);
     /* chunks and looks of pixel q at ptab[q], ptab[PPW + q]: read back with
        a uniform address when q is only known at run time (a register
@@ -367,21 +368,20 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
             }
             base += __popcll(m64);
         }
-        if (!BF16) {
-            /* FP16: each entry also carries its pixel's exponent below the
-               window's largest accepted one (LIST_SHIFT_BITS), by which
-               step() rescales its hi and lo parts */
-            emax = gmaxi<WAVE>(emax);
-            wave_sync();
-            for (int t = lane; t < base; t += WAVE) {
-                const int cand = lq[t];
-                const int sh = min(emax - (int)ex[cand], LIST_SHIFT_MAX);
-
-                lq[t] = (int)((unsigned)cand | ((unsigned)sh << LIST_SHIFT_BITS));
-            }
-        }
         const int lpad = (base + 31) & ~31;
 
+        if (!BF16) {
+            /* FP16: each candidate's factor 2^-(its exponent below the
+               window's largest accepted one), by which step() rescales its
+               hi and lo parts */
+            uint32_t *fq = fac + q * lstride;
+
+            emax = gmaxi<WAVE>(emax);
+            wave_sync();
+            for (int t = lane; t < lpad; t += WAVE)
+                fq[t] = t < base ? f16_pow2_neg_pair(min(emax - (int)ex[lq[t]], LIST_SHIFT_MAX))
+                                 : F16_ONE_PAIR;
+        }
         for (int t = base + lane; t < lpad; t += WAVE)
             lq[t] = kp.npad;
         nch[q] = lpad >> 5;
@@ -431,7 +431,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const uint2 *row = hl + (size_t)((unsigned)idx[e] & LIST_INDEX_MASK) * NB + i16;
+            const uint2 *row = hl + (size_t)(unsigned)idx[e] * NB + i16;
 
 #pragma unroll
             for (int bk = 0; bk < NBK; bk++)
@@ -459,13 +459,13 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
             int cq, off;
 
             locate(c, cq, off);
-            const int4 *lp = reinterpret_cast<const int4 *>(list + cq * lstride + off * 32 + g * 8);
-            const int4 i0 = lp[0], i1 = lp[1];
-            const int ent[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+            const uint4 *fp = reinterpret_cast<const uint4 *>(fac + cq * lstride + off * 32 + g * 8);
+            const uint4 f0 = fp[0], f1 = fp[1];
+            const uint32_t fe[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
 
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const uint32_t f = f16_pow2_neg_pair((int)((unsigned)ent[e] >> LIST_SHIFT_BITS));
+                const uint32_t f = fe[e];
 
 #pragma unroll
                 for (int bk = 0; bk < NBK; bk++)

@@ -723,7 +723,9 @@ __device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
 template <int N, int G, int CPL, bool EXACT>
 __device__ __forceinline__ void hermitian_eigenvector(float2 (&A)[CPL][N],
                                                       EigLds<N> &sh, int idx,
-                                                      int lg, float2 (&z)[CPL])
+                                                      int lg, float2 (&z)[CPL],
+                                                      float *lam_out = nullptr,
+                                                      float *norm_out = nullptr)
 {
     float norm;
 
@@ -733,6 +735,34 @@ __device__ __forceinline__ void hermitian_eigenvector(float2 (&A)[CPL][N],
     const float lambda = tridiag_eigenvalue<N, G, true>(sh, idx, lg, &norm);
     tridiag_eigenvector<N, EXACT>(sh, lambda, norm, lg);
     back_transform<N, G, CPL>(A, sh, lg, z);
+    if (lam_out) {
+        *lam_out = lambda;
+        *norm_out = norm;
+    }
+}
+
+/* Whether the FP32 eigenvector of the smallest eigenvalue lambda of the
+   tridiagonal in sh (||.|| <= norm) may be off by more than REFINE_TOL:
+   its error is about eps_32 norm / gap, so it is when a second eigenvalue
+   lies below lambda + eps_32 norm / REFINE_TOL (a Sturm count there). */
+#define REFINE_TOL 1.0e-5f
+template <int N>
+__device__ __forceinline__ bool close_second_eigenvalue(const EigLds<N> &sh, float lambda,
+                                                        float norm)
+{
+    const float x = lambda + (0x1p-24f / REFINE_TOL) * norm;
+    const float pivmin = 1.0e-30f + 1.0e-7f * norm * 1.0e-7f;
+    float qv = sh.d[0] - x;
+    int cnt = qv < 0.0f;
+
+#pragma unroll
+    for (int i = 1; i < N; i++) {
+        if (fabsf(qv) < pivmin)
+            qv = -pivmin;
+        qv = sh.d[i] - x - sh.e2[i - 1] / qv;
+        cnt += qv < 0.0f;
+    }
+    return cnt >= 2;
 }
 
 /* T_rc of the pixel from the packed strict upper triangle. */

@@ -141,18 +141,20 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
     }
     /* (only live groups store an eigenvector: a group past the row's end
        has bp of the row's first pixel) */
+    bool refine = false;
     const float gam = link_group<N, G, EMI, INV_CHOLESKY>(
         kp, tl + gi * NT, est, lg, sh[gi], *em, deferred,
-        EMI && live ? q.z + (size_t)bp * N : nullptr);
+        EMI && live ? q.z + (size_t)bp * N : nullptr, EMI ? &refine : nullptr);
 
     if constexpr (EMI) {
         const int slot = queue_slot<G>(q.n, lg == 0 && deferred, gi);
+        const bool fine = live && !deferred && refine && !__builtin_isnan(gam);
+        const int s3 = queue_slot<G>(q.n + 2, lg == 0 && fine, gi);
 
-        if (lg == 0 && live) {
-            if (deferred)
-                q.pix[slot] = bp;
-            q.how[bp] = (!deferred && !__builtin_isnan(gam)) ? 1 : 0;
-        }
+        if (lg == 0 && live && deferred)
+            q.pix[slot] = bp;
+        if (lg == 0 && fine)
+            q.ref[s3] = bp | (1 << REFINE_KIND_SHIFT);
     }
     if (live && lg == 0 && !deferred)
         coh[p] = gam;
@@ -182,21 +184,22 @@ __global__ __launch_bounds__(64, 1) void k_emi_queue(KParams kp, EmiQueue q, int
         const int slot = base + gi;
         const bool live = slot < total;
         const int bp = live ? q.pix[slot] : 0;
-        bool deferred;
+        bool deferred, refine = false;
         const float gam = link_group<N, G, 1, INV_GAUSS_JORDAN>(
             kp, q.t + (size_t)bp * NT, live, lg, sh[gi], *em, deferred,
-            live ? q.z + (size_t)bp * N : nullptr);
+            live ? q.z + (size_t)bp * N : nullptr, &refine);
         const int s2 = queue_slot<G>(q.n + 1, lg == 0 && live && deferred, gi);
+        const bool fine = live && !deferred && refine && !__builtin_isnan(gam);
+        const int s3 = queue_slot<G>(q.n + 2, lg == 0 && fine, gi);
 
         if (live && lg == 0) {
-            if (deferred) {
+            if (deferred)
                 q.next[s2] = bp;
-            }
-            else {
+            else
                 coh[pbase + bp] = gam;
-                q.how[bp] = __builtin_isnan(gam) ? 0 : 2;
-            }
         }
+        if (lg == 0 && fine)
+            q.ref[s3] = bp | (2 << REFINE_KIND_SHIFT);
         if (live && !deferred)
             store_phases<N, G>(kp, sh[gi], pbase + bp, lg, !__builtin_isnan(gam));
     }
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(64, 1) void k_emi_queue(KParams kp, EmiQueue q, int
 
 /* Queue 2 (pixels whose |T| the condition bound leaves to the eigen
    pseudo-inverse): the FP64 Jacobi pseudo-inverse, then the linking (not
-   refined: how stays 0). */
+   refined: the refinement's inverse is not that pseudo-inverse). */
 template <int N>
 __global__ __launch_bounds__(64, 1) void k_emi_jacobi(KParams kp, EmiQueue q, int pbase,
                                                       float *__restrict__ coh)
@@ -232,11 +235,12 @@ __global__ __launch_bounds__(64, 1) void k_emi_jacobi(KParams kp, EmiQueue q, in
     }
 }
 
-/* The FP64 refinement (refine.h) of every linked EMI pixel of the band
-   whose |T|^-1 is the inverse (how 1 or 2): one inverse iteration from its
-   FP32 eigenvector, phasors and gamma in FP64, which replace the FP32
-   results. G = 32 lanes a pixel (two pixels a wavefront) up to 32 dates,
-   64 above; a fixed grid strides over the band. */
+/* The FP64 refinement (refine.h) of the band's pixels in queue 3 (linked,
+   |T|^-1 the inverse, a second eigenvalue of M close to the smallest): one
+   inverse iteration from the FP32 eigenvector, phasors and gamma in FP64,
+   which replace the FP32 results. G = 32 lanes a pixel (two pixels a
+   wavefront) up to 32 dates, 64 above; a fixed grid strides over the
+   queue. */
 template <int N> struct RefineShape {
     static constexpr int G = N <= 32 ? 32 : 64;
     static constexpr int PPW = WAVE / G;
@@ -245,7 +249,7 @@ template <int N> struct RefineShape {
 
 template <int N>
 __global__ __launch_bounds__(64, 1) void k_emi_refine(KParams kp, EmiQueue q, int pbase,
-                                                      int npix, float *__restrict__ coh)
+                                                      float *__restrict__ coh)
 {
     typedef RefineShape<N> R;
     constexpr int G = R::G, PPW = R::PPW, NT = N * (N - 1) / 2;
@@ -253,10 +257,13 @@ __global__ __launch_bounds__(64, 1) void k_emi_refine(KParams kp, EmiQueue q, in
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
     RefineLds<N> &sh = *reinterpret_cast<RefineLds<N> *>(lds + gi * R::STRIDE);
     const size_t plane = (size_t)kp.rows * kp.cols;
+    const int total = q.n[2];
 
-    for (int base = blockIdx.x * PPW; base < npix; base += gridDim.x * PPW) {
-        const int bp = base + gi;
-        const int how = bp < npix ? q.how[bp] : 0;
+    for (int base = blockIdx.x * PPW; base < total; base += gridDim.x * PPW) {
+        const int k = base + gi;
+        const int e = k < total ? q.ref[k] : 0;
+        const int how = e >> REFINE_KIND_SHIFT;
+        const int bp = e & ((1 << REFINE_KIND_SHIFT) - 1);
         double g;
 
         if (how == 0)
@@ -315,8 +322,8 @@ hipError_t launch_emi_queue_n(const KParams &kp, const EmiQueue &q, int pbase, i
                        s, kp, q, pbase, coh);
     if ((e = hipGetLastError()) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_emi_refine<N>, dim3(queue_grid(cap, R::PPW, 8192)), dim3(WAVE), lr,
-                       s, kp, q, pbase, cap, coh);
+    hipLaunchKernelGGL(k_emi_refine<N>, dim3(queue_grid(cap, R::PPW, 4096)), dim3(WAVE), lr,
+                       s, kp, q, pbase, coh);
     return hipGetLastError();
 }
 
@@ -331,7 +338,7 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
     const int scratch = link_scratch_bytes<N>(kp, precision);
     const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::PPW * S::CH_STRIDE : 0);
 
-    if (kp.emi && !(q.t && q.z && q.how && q.pix && q.n && q.next))
+    if (kp.emi && !(q.t && q.z && q.ref && q.pix && q.n && q.next))
         return hipErrorInvalidValue;
 #define TCOH_LINK_LAUNCH(EMI)                                                   \
     do {                                                                        \

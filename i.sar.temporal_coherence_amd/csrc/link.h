@@ -22,7 +22,8 @@ __device__ __forceinline__ float link_group(const KParams &kp,
                                             const float2 *tp, bool est,
                                             int lg, EigLds<N> &sh,
                                             EmiLds<N> &em, bool &deferred,
-                                            float2 *zraw = nullptr)
+                                            float2 *zraw = nullptr,
+                                            bool *refine = nullptr)
 {
     deferred = false;
     constexpr int CPL = (N + G - 1) / G;
@@ -123,8 +124,13 @@ __device__ __forceinline__ float link_group(const KParams &kp,
                 A[m][r] = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
             }
         }
-        hermitian_eigenvector<N, G, CPL, true>(A, sh, 0, lg, z);
-        /* the eigenvector itself, for the FP64 refinement (refine.h) */
+        float lam, norm;
+
+        hermitian_eigenvector<N, G, CPL, true>(A, sh, 0, lg, z, &lam, &norm);
+        /* pixels whose FP32 eigenvector may be off by more than REFINE_TOL
+           are refined in FP64 (refine.h), from the eigenvector itself */
+        if (refine)
+            *refine = close_second_eigenvalue<N>(sh, lam, norm);
         if (zraw) {
 #pragma unroll
             for (int m = 0; m < CPL; m++)

@@ -6,6 +6,7 @@ call changes."""
 import numpy as np
 import pytest
 
+import synth
 from conftest import strip_of
 
 pytestmark = pytest.mark.gpu
@@ -25,10 +26,10 @@ CASES = {
 
 def case_stack(tcoh, c):
     if c["g"] is None:
-        reg, params = tcoh.synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
+        reg, params = synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
                                                 gamma0=(0.6, 0.99))
-        return tcoh.synth.scene_stack(30, 40, 40, reg, params, seed=0)
-    return tcoh.synth.ds_stack(c["n"], c["g"], seed=c["seed"])
+        return synth.scene_stack(30, 40, 40, reg, params, seed=0)
+    return synth.ds_stack(c["n"], c["g"], seed=c["seed"])
 
 
 @pytest.mark.parametrize("name", list(CASES))

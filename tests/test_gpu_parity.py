@@ -8,6 +8,7 @@ FP64 oracle, with the same NaN pattern (reference test:53-62, 309-315).
 import numpy as np
 import pytest
 
+import synth
 from conftest import strip_of
 from test_oracle import CASES, TOLERANCE
 
@@ -90,7 +91,7 @@ def check_against_oracles(tcoh, cref, fp64, stack, opts, precision, tol=TOLERANC
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
 def test_reference_cases(tcoh, cref, fp64, case, precision):
-    stack = tcoh.synth.ds_stack(case["n"], case["g"], seed=case["seed"])
+    stack = synth.ds_stack(case["n"], case["g"], seed=case["seed"])
     check_against_oracles(tcoh, cref, fp64, stack, case["opts"], precision,
                           tol_for(precision))
 
@@ -119,7 +120,7 @@ EDGE_CASES = [
                          ids=["n%d_%dx%d" % (c["n"], *c["opts"].get("window", (21, 7)))
                               for c in EDGE_CASES])
 def test_shape_edges(tcoh, cref, fp64, case, precision):
-    stack = tcoh.synth.ds_stack(case["n"], case["g"], seed=case["seed"])
+    stack = synth.ds_stack(case["n"], case["g"], seed=case["seed"])
     coh, count, _ = check_against_oracles(tcoh, cref, fp64, stack, case["opts"],
                                           precision, tol_for(precision))
     assert np.isfinite(coh).sum() > 200
@@ -127,7 +128,7 @@ def test_shape_edges(tcoh, cref, fp64, case, precision):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_nulls_and_zero_samples(tcoh, cref, fp64, precision):
-    stack = tcoh.synth.ds_stack(5, 0.8, seed=6)
+    stack = synth.ds_stack(5, 0.8, seed=6)
     stack[2, 10:13, 4:9] = np.nan
     stack[4, 20, 20] = 0
     coh, count, _ = check_against_oracles(tcoh, cref, fp64, stack, dict(min_shp=5),
@@ -141,7 +142,7 @@ def test_nulls_and_zero_samples(tcoh, cref, fp64, precision):
 def test_strips_are_seamless(tcoh, precision):
     """reference test:118-135: a tiny memory budget gives several strips and
     the same maps as one strip."""
-    stack = tcoh.synth.ds_stack(8, 0.6, rows=30, cols=400, seed=4)
+    stack = synth.ds_stack(8, 0.6, rows=30, cols=400, seed=4)
     one = tcoh.temporal_coherence(stack, window=(7, 5), min_shp=10, precision=precision)
     many = tcoh.temporal_coherence(stack, window=(7, 5), min_shp=10, memory=1,
                                    precision=precision)
@@ -155,7 +156,7 @@ def test_coherent_and_incoherent(tcoh):
     for est in ("evd", "emi"):
         means = {}
         for g in (0.95, 0.0):
-            coh, _, _ = tcoh.temporal_coherence(tcoh.synth.ds_stack(10, g, seed=5),
+            coh, _, _ = tcoh.temporal_coherence(synth.ds_stack(10, g, seed=5),
                                                 estimator=est)
             means[g] = np.nanmean(coh)
         assert means[0.95] > 0.9 and means[0.0] < 0.5
@@ -164,7 +165,7 @@ def test_coherent_and_incoherent(tcoh):
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_small_scene(tcoh, cref, fp64, precision):
     """N=10 dates, 256 x 256, four coherence quadrants, KS, EVD, min_shp 10."""
-    stack = tcoh.synth.config_small(seed=0)
+    stack = synth.config_small(seed=0)
     check_against_oracles(tcoh, cref, fp64, stack, dict(min_shp=10), precision,
                           tol_for(precision))
 
@@ -173,8 +174,8 @@ def test_small_scene(tcoh, cref, fp64, precision):
 def test_depth30_scene_crop(tcoh, cref, fp64, precision):
     """N=30, Voronoi decorrelation scene crop (96 x 96), KS, EVD, min_shp 30,
     bias correction, consecutive pairs."""
-    reg, params = tcoh.synth.config_voronoi(n=30, rows=96, cols=96, seed=1)
-    stack = tcoh.synth.scene_stack(30, 96, 96, reg, params, seed=1)
+    reg, params = synth.config_voronoi(n=30, rows=96, cols=96, seed=1)
+    stack = synth.scene_stack(30, 96, 96, reg, params, seed=1)
     check_against_oracles(tcoh, cref, fp64, stack,
                           dict(min_shp=30, bias=True, pairs="consecutive"), precision,
                           tol_for(precision))
@@ -187,9 +188,9 @@ def test_deep_stacks(tcoh, cref, fp64, n, estimator, precision):
     """Stacks deeper than 32 dates (one pixel per wavefront, 64 date slots;
     N=45 and N=60-64 are the depths of the batched and full-geometry
     workloads): Voronoi scene crop, KS, bias, consecutive pairs."""
-    reg, params = tcoh.synth.config_voronoi(n=n, rows=36, cols=30, seed=n,
+    reg, params = synth.config_voronoi(n=n, rows=36, cols=30, seed=n,
                                             gamma0=(0.3, 0.9))
-    stack = tcoh.synth.scene_stack(n, 36, 30, reg, params, seed=n)
+    stack = synth.scene_stack(n, 36, 30, reg, params, seed=n)
     check_against_oracles(tcoh, cref, fp64, stack,
                           dict(min_shp=n, estimator=estimator, bias=True,
                                pairs="consecutive"), precision, tol_for(precision))
@@ -199,9 +200,9 @@ def emi_fallback_scene(tcoh):
     """N=30 crop of the AD+EMI workload (gamma0 ~ U(0.6, 0.99)): |T| fails
     the Cholesky floor at many pixels, which then take the Jacobi
     pseudo-inverse (tcoh.cl:657-687)."""
-    reg, params = tcoh.synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
+    reg, params = synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
                                             gamma0=(0.6, 0.99))
-    return tcoh.synth.scene_stack(30, 40, 40, reg, params, seed=0)
+    return synth.scene_stack(30, 40, 40, reg, params, seed=0)
 
 
 @pytest.mark.parametrize("precision", ["plain", "f16"])
@@ -230,7 +231,7 @@ def test_geometry_grid_matches_oracle(tcoh, cref):
     (geolocation steps < 1e-6 m: 4 pi / lambda x 1e-6 m = 2.3e-4 rad);
     asserted 1e-6 rad, on three heights and a reference date in the
     middle."""
-    orbits, scene, ref = tcoh.synth.geometry_case(n=6, rows=400, cols=700, seed=2)
+    orbits, scene, ref = synth.geometry_case(n=6, rows=400, cols=700, seed=2)
     got, hs = tcoh.api.geometry_grid(orbits, scene, ref, 10.0, 900.0)
     want, ohs, status = cref.geometry_grid(orbits, scene, ref, 10.0, 900.0)
     assert status == 0 and got.shape == want.shape and np.array_equal(hs, ohs)
@@ -250,7 +251,7 @@ def test_geometry_from_ingested_orbits(tcoh, cref, tmp_path):
     FP64 restatement on the same orbits to 1e-6 rad, and the two orbit
     sources agree to 1e-3 rad (the EOF's 1 um rounding is 4.5e-4 rad at
     C-band)."""
-    api, syn = tcoh.api, tcoh.synth
+    api, syn = tcoh.api, synth
     api.lib().tcoh_set_error_mode(1)
     orbits, scene, ref = syn.geometry_case(n=4, rows=300, cols=500, seed=4)
     t0s = [o[0][len(o[0]) // 2] for o in orbits]
@@ -294,11 +295,11 @@ def test_linked_phases(tcoh, cref, fp64, scene, precision):
     """The linked phases (tcoh_run_phases) against the FP64 oracle's unit
     phasors: the per-date wrapped phase error, max and RMS."""
     if scene == "n30_crop":
-        reg, params = tcoh.synth.config_voronoi(n=30, rows=64, cols=64, seed=1)
-        stack = tcoh.synth.scene_stack(30, 64, 64, reg, params, seed=1)
+        reg, params = synth.config_voronoi(n=30, rows=64, cols=64, seed=1)
+        stack = synth.scene_stack(30, 64, 64, reg, params, seed=1)
         opts = dict(min_shp=30, bias=True, pairs="all")
     elif scene == "n10_quadrants":
-        stack = tcoh.synth.config_small(seed=0)[:, 96:160, 96:160]
+        stack = synth.config_small(seed=0)[:, 96:160, 96:160]
         opts = dict(min_shp=10)
     else:
         stack = emi_fallback_scene(tcoh)
@@ -339,7 +340,7 @@ def test_geometry_grid_matches_exact_orbits(tcoh):
     C-band); asserted 1e-5 rad."""
     import math
     from test_host import _analytic_node_phases
-    orbits, scene, ref = tcoh.synth.geometry_case(n=4, rows=100, cols=200, seed=5)
+    orbits, scene, ref = synth.geometry_case(n=4, rows=100, cols=200, seed=5)
     radius, omega = 6378137.0 + 700e3, 7500.0 / (6378137.0 + 700e3)
 
     def exact(t0, b):

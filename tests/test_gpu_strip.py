@@ -2,15 +2,20 @@
 oracle's restatement of read_strip (main.c:52-127), through the C ABI.
 
 Validity and amplitudes must match bit for bit, and so must the samples
-without a geometric phase. With one, the samples are rotated by the same
-FP64 phase (the restatement of geometry.c:256-291 evaluated in the same
-order) through the device's FP64 sin/cos instead of the host's: both are
-within an ulp of the exact value, so a rotated sample may differ by one
-float ulp; asserted: at most one ulp, and nearly all identical.
+without a geometric phase. With one, the samples are rotated by the FP64
+phase of the grid (geometry.c:256-291, here evaluated in another form, so
+within a few ulp of the oracle's: 1e-11 rad at phases of 1e4 rad) through
+the device's FP64 sin/cos instead of the host's: a rotated sample may then
+differ from the oracle's by the float rounding of its components, which is
+small against the sample's magnitude though not always against a component
+near zero; asserted: within 2.5e-7 of the magnitude, and nearly all
+identical.
 """
 
 import numpy as np
 import pytest
+
+import synth
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +24,7 @@ WINDOW = (9, 5)
 
 def planes(tcoh, n, rows, cols, seed):
     """I/Q date planes with nulls and zero samples planted."""
-    stack = tcoh.synth.ds_stack(n, 0.6, rows=rows, cols=cols, seed=seed)
+    stack = synth.ds_stack(n, 0.6, rows=rows, cols=cols, seed=seed)
     i = np.ascontiguousarray(stack.real, np.float32)
     q = np.ascontiguousarray(stack.imag, np.float32)
     rng = np.random.default_rng(seed + 100)
@@ -80,7 +85,7 @@ def test_strip_with_geometric_phase(tcoh, cref, n, elevation):
     rows, cols = 37, 45
     ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
     i, q = planes(tcoh, n, rows, cols, seed=3 * n)
-    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, seed=4)
+    orbits, scene, ref = synth.geometry_case(n=n, rows=rows, cols=cols, seed=4)
     grid, hs, status = cref.geometry_grid(orbits, scene, ref, 0.0, 500.0)
     assert status == 0 and grid.shape[2] == 3
     rng = np.random.default_rng(5)
@@ -96,9 +101,12 @@ def test_strip_with_geometric_phase(tcoh, cref, n, elevation):
     want = cref.prepare_strip(i, q, row0, prow, hr, ph, en)
     assert np.array_equal(got[2], want[2])
     assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
-    d = ulps(got[0], want[0])
-    assert d.max() <= 1, d.max()
-    assert np.mean(d == 0) > 0.999
+    gc = got[0].reshape(-1, 2).astype(np.float64)
+    wc = want[0].reshape(-1, 2).astype(np.float64)
+    mag = np.hypot(wc[:, 0], wc[:, 1])
+    rel = np.hypot(*(gc - wc).T) / np.where(mag > 0, mag, 1.0)
+    assert rel.max() <= 2.5e-7, rel.max()
+    assert np.mean(rel == 0) > 0.99
     # the rotation is real: the rotated samples differ from the raw ones
     raw = cref.prepare_strip(i, q, row0, prow, hr)
     assert not np.array_equal(raw[0], want[0])
@@ -182,7 +190,7 @@ def test_region_stream_with_geometric_phase(tcoh, cref):
     n, rows, cols = 6, 58, 40
     ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
     i, q = planes(tcoh, n, rows, cols, seed=41)
-    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, seed=6)
+    orbits, scene, ref = synth.geometry_case(n=n, rows=rows, cols=cols, seed=6)
     grid, hs, status = cref.geometry_grid(orbits, scene, ref, 0.0, 400.0)
     assert status == 0
     heights = np.random.default_rng(7).uniform(0.0, 400.0, (rows, cols))
@@ -258,16 +266,16 @@ def test_depth60_with_geometric_phase(tcoh, cref, fp64, precision):
     n, rows, cols = 60, 34, 30
     window = (21, 7)
     ha, hr = window[0] // 2, window[1] // 2
-    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=rows, cols=cols, ref=0, seed=7)
+    orbits, scene, ref = synth.geometry_case(n=n, rows=rows, cols=cols, ref=0, seed=7)
     grid, hs = tcoh.api.geometry_grid(orbits, scene, ref, 0.0, 500.0)
     g = tcoh.api.make_geometry(grid, hs)
     yy, xx = np.mgrid[0:rows, 0:cols]
     dem = 250.0 + 240.0 * np.sin(yy / 7.0) * np.cos(xx / 5.0)
     dem[5, 6] = np.nan
     phi = cref.pixel_phases(grid, hs, rows, cols, elevation=dem)
-    reg, params = tcoh.synth.config_voronoi(n=n, rows=rows, cols=cols, seed=3,
+    reg, params = synth.config_voronoi(n=n, rows=rows, cols=cols, seed=3,
                                             gamma0=(0.3, 0.9))
-    obs = tcoh.synth.scene_stack(n, rows, cols, reg, params, seed=3) * np.exp(-1j * phi)
+    obs = synth.scene_stack(n, rows, cols, reg, params, seed=3) * np.exp(-1j * phi)
     i = np.ascontiguousarray(obs.real, np.float32)
     q = np.ascontiguousarray(obs.imag, np.float32)
     prow, pc = rows + 2 * ha, cols + 2 * hr

@@ -8,6 +8,7 @@ import re
 import numpy as np
 import pytest
 
+import synth
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "tcoh_hip.h")
@@ -120,7 +121,7 @@ def test_settings_derivation(tcoh):
 def test_prepare_strip_matches_oracle(tcoh, cref):
     """read_strip (main.c:52-127): validity, sorted sqrtf amplitudes and the
     double-precision rotation, bit for bit."""
-    stack = tcoh.synth.ds_stack(6, 0.6, rows=20, cols=17, seed=11)
+    stack = synth.ds_stack(6, 0.6, rows=20, cols=17, seed=11)
     stack[1, 3:5, 2:6] = np.nan
     stack[4, 7, 7] = 0
     rng = np.random.default_rng(0)
@@ -225,7 +226,7 @@ def test_geometry_oracle_matches_exact_orbits(tcoh, cref):
     the 8-point Lagrange interpolation of 10 s state vectors and the Newton
     stopping rules, together well below a micrometre of range on a circular
     orbit (4 pi / lambda x 1 um = 2.3e-4 rad at C-band); asserted: 1e-5 rad."""
-    orbits, scene, ref = tcoh.synth.geometry_case(n=4, rows=100, cols=200, seed=5)
+    orbits, scene, ref = synth.geometry_case(n=4, rows=100, cols=200, seed=5)
     radius, omega = 6378137.0 + 700e3, 7500.0 / (6378137.0 + 700e3)
 
     def exact(t0, b):

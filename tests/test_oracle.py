@@ -6,6 +6,7 @@ import math
 import numpy as np
 import pytest
 
+import synth
 from conftest import strip_of
 
 TOLERANCE = 1e-4  # largest |d gamma|, |d pair| vs FP64 (reference test:27)
@@ -241,7 +242,7 @@ def run_oracles(cref, fp64, tcoh_synth, case):
 
 @pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
 def test_fp32_restatement_within_tolerance_of_fp64(cref, fp64, tcoh, case):
-    s, coh, count, pairs, c64, p64, flips = run_oracles(cref, fp64, tcoh.synth, case)
+    s, coh, count, pairs, c64, p64, flips = run_oracles(cref, fp64, synth, case)
     assert flips == 0
     assert np.array_equal(np.isnan(coh), np.isnan(c64))
     assert np.nanmax(np.abs(coh - c64)) <= TOLERANCE
@@ -253,7 +254,7 @@ def test_fp32_restatement_within_tolerance_of_fp64(cref, fp64, tcoh, case):
 def test_bias_unlinked_dates_give_nan_with_finite_pairs(cref, fp64, tcoh):
     """reference test:357-378: with -b, unlinked dates -> NaN gamma, pairs
     still defined."""
-    s, coh, count, pairs, c64, p64, _ = run_oracles(cref, fp64, tcoh.synth, CASES[3])
+    s, coh, count, pairs, c64, p64, _ = run_oracles(cref, fp64, synth, CASES[3])
     unlinked = np.isnan(coh) & (count >= s.min_looks)
     assert unlinked.any()
     assert np.isfinite(pairs[0][unlinked]).all()
@@ -264,7 +265,7 @@ def test_bias_pairs_are_sqrt_m2c(cref, fp64, tcoh):
     the parity tests use to bound pair errors near the threshold."""
     case = CASES[3]
     n, opts = case["n"], case["opts"]
-    stack = tcoh.synth.ds_stack(n, case["g"], seed=case["seed"])
+    stack = synth.ds_stack(n, case["g"], seed=case["seed"])
     s = cref.settings(n, window=opts["window"], bias=True, min_shp=opts["min_shp"],
                       pairs="all")
     slc, amp, valid, prow, pcols = strip_of(stack, opts["window"])
@@ -303,7 +304,7 @@ def test_coherent_and_incoherent(cref, fp64, tcoh):
     """reference test:138-151."""
     means = {}
     for g in (0.95, 0.0):
-        stack = tcoh.synth.ds_stack(10, g, seed=5)
+        stack = synth.ds_stack(10, g, seed=5)
         s = cref.settings(10)
         slc, amp, valid, prow, pcols = strip_of(stack, (21, 7))
         coh, *_ = cref.run(s, pcols, slc, amp, valid, prow, 30, 24)
@@ -313,7 +314,7 @@ def test_coherent_and_incoherent(cref, fp64, tcoh):
 
 def test_nulls_and_zero_samples(cref, tcoh):
     """reference test:154-169."""
-    stack = tcoh.synth.ds_stack(5, 0.8, seed=6)
+    stack = synth.ds_stack(5, 0.8, seed=6)
     stack[2, 10:13, 4:9] = np.nan
     stack[4, 20, 20] = 0
     s = cref.settings(5, min_shp=5)
@@ -329,9 +330,9 @@ def test_emi_fallback_scene_reaches_the_jacobi_path(cref, fp64, tcoh):
     """The GPU test scene of the EMI fallback really holds pixels whose |T|
     fails the Cholesky floor (so the Jacobi pseudo-inverse is exercised),
     and others that pass it."""
-    reg, params = tcoh.synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
+    reg, params = synth.config_voronoi(n=30, rows=40, cols=40, seed=0,
                                             gamma0=(0.6, 0.99))
-    stack = tcoh.synth.scene_stack(30, 40, 40, reg, params, seed=0)
+    stack = synth.scene_stack(30, 40, 40, reg, params, seed=0)
     slc, amp, valid, prow, pcols = strip_of(stack, (21, 7))
     s = cref.settings(30, shp_test="ad", min_shp=20, estimator="emi", bias=True,
                       pairs="all")

@@ -17,6 +17,8 @@ import os
 import numpy as np
 import pytest
 
+import synth
+
 EPOCH = dt.datetime(1970, 1, 1)
 T2023 = (dt.datetime(2023, 1, 12, 5, 43, 21, 250000) - EPOCH).total_seconds()
 
@@ -158,12 +160,12 @@ def test_json_escapes_follow_the_reference(api):
 def _case(tcoh, half_span=60.0, n=3):
     """Orbits of a 3-date stack shifted to January 2023, the reference
     date's scene and every date's first line time."""
-    orbits, scene, ref = tcoh.synth.geometry_case(n=n, rows=200, cols=300, seed=3)
+    orbits, scene, ref = synth.geometry_case(n=n, rows=200, cols=300, seed=3)
     t0s = [o[0][len(o[0]) // 2] for o in orbits]
     base = [o[1][0, 1] for o in orbits]
     shift = T2023 - t0s[ref]
     t0s = [t + shift for t in t0s]
-    orbits = tcoh.synth.circular_orbits(t0s, base, half_span=half_span)
+    orbits = synth.circular_orbits(t0s, base, half_span=half_span)
     scene = dict(scene, t_first=t0s[ref])
     return orbits, scene, ref, t0s
 
@@ -171,17 +173,17 @@ def _case(tcoh, half_span=60.0, n=3):
 def test_annotation_state_vectors_are_read_exactly(tcoh, api):
     orbits, scene, ref, t0s = _case(tcoh)
     for k, o in enumerate(orbits):
-        meta = tcoh.synth.s1_metadata(o, scene, t0s[k])
+        meta = synth.s1_metadata(o, scene, t0s[k])
         t, pos, vel, source = api.orbit_from_annotation(meta, f"s{k}")
         assert source == "annotation"
         # times: the microsecond text of each state vector
-        want_t = [api.parse_utc(tcoh.synth.iso_utc(x)) for x in o[0]]
+        want_t = [api.parse_utc(synth.iso_utc(x)) for x in o[0]]
         assert np.array_equal(t, want_t)
         assert np.max(np.abs(t - o[0])) < 1e-6
         # numbers: JSON carries them at full precision
         assert np.array_equal(pos, o[1]) and np.array_equal(vel, o[2])
         start, stop = api.acquisition_span(meta)
-        assert start == api.parse_utc(tcoh.synth.iso_utc(t0s[k]))
+        assert start == api.parse_utc(synth.iso_utc(t0s[k]))
         assert stop - start == pytest.approx((scene["rows"] - 1) * scene["dt"], abs=2e-6)
 
 
@@ -190,7 +192,7 @@ def test_eof_state_vectors_are_read_exactly(tcoh, api):
     [t0, t1] are skipped."""
     orbits, scene, ref, t0s = _case(tcoh, half_span=600.0)
     o = orbits[0]
-    text = tcoh.synth.eof_text(o)
+    text = synth.eof_text(o)
     t, pos, vel, source = api.orbit_from_eof(text, -1e30, 1e30, "POEORB")
     assert source == "POEORB" and len(t) == len(o[0]) == 121
     assert np.array_equal(pos, np.array([[float(f"{x:.6f}") for x in r] for r in o[1]]))
@@ -214,14 +216,14 @@ def test_eof_and_annotation_give_the_same_orbit(tcoh, api, cref):
     import ctypes as C
 
     orbits_ann, scene, ref, t0s = _case(tcoh)
-    orbits_eof = tcoh.synth.circular_orbits(t0s, [o[1][0, 1] for o in orbits_ann],
+    orbits_eof = synth.circular_orbits(t0s, [o[1][0, 1] for o in orbits_ann],
                                             half_span=600.0)
     got_ann, got_eof = [], []
     for k in range(len(t0s)):
-        meta = tcoh.synth.s1_metadata(orbits_ann[k], scene, t0s[k])
+        meta = synth.s1_metadata(orbits_ann[k], scene, t0s[k])
         start, stop = api.acquisition_span(meta)
         a = api.orbit_from_annotation(meta)
-        e = api.orbit_from_eof(tcoh.synth.eof_text(orbits_eof[k]), start - 120, stop + 120)
+        e = api.orbit_from_eof(synth.eof_text(orbits_eof[k]), start - 120, stop + 120)
         got_ann.append(a[:3])
         got_eof.append(e[:3])
         for tq in np.linspace(start, stop, 7):
@@ -245,14 +247,14 @@ def test_eof_and_annotation_give_the_same_orbit(tcoh, api, cref):
 def test_annotation_coverage_and_errors(tcoh, api):
     orbits, scene, ref, t0s = _case(tcoh)
     o = orbits[0]
-    meta = tcoh.synth.s1_metadata(o, scene, t0s[0])
+    meta = synth.s1_metadata(o, scene, t0s[0])
     # fewer than 8 vectors, or not covering the acquisition
     short = tuple(x[:7] for x in o)
     with pytest.raises(RuntimeError, match="does not cover"):
-        api.orbit_from_annotation(tcoh.synth.s1_metadata(short, scene, t0s[0]))
+        api.orbit_from_annotation(synth.s1_metadata(short, scene, t0s[0]))
     late = tuple(x[7:] for x in o)
     with pytest.raises(RuntimeError, match="does not cover"):
-        api.orbit_from_annotation(tcoh.synth.s1_metadata(late, scene, t0s[0]))
+        api.orbit_from_annotation(synth.s1_metadata(late, scene, t0s[0]))
     d = json.loads(meta)
     d["swath"]["orbit_state_vectors"][3]["velocity"][1] = "fast"
     with pytest.raises(RuntimeError, match="Invalid orbit state vector 3"):
@@ -285,7 +287,7 @@ def test_orbit_file_choice(tcoh, api, tmp_path):
     suffix, validity covering the acquisition with 120 s of margin, the
     latest production among several; precise before restituted before
     annotation."""
-    syn = tcoh.synth
+    syn = synth
     orbits, scene, ref, t0s = _case(tcoh, half_span=600.0)
     o = orbits[0]
     ann = tuple(x[54:67] for x in o)  # 13 vectors, t0 +- 60 s
@@ -331,9 +333,9 @@ def test_orbit_file_choice(tcoh, api, tmp_path):
 
 def test_scene_from_metadata(tcoh, api):
     orbits, scene, ref, t0s = _case(tcoh)
-    metas = [tcoh.synth.s1_metadata(o, scene, t0s[k]) for k, o in enumerate(orbits)]
+    metas = [synth.s1_metadata(o, scene, t0s[k]) for k, o in enumerate(orbits)]
     sc = api.scene_from_metadata(metas, ref, 12.0, 34.0, 200, 300)
-    assert sc["t_first"] == api.parse_utc(tcoh.synth.iso_utc(t0s[ref]))
+    assert sc["t_first"] == api.parse_utc(synth.iso_utc(t0s[ref]))
     for key in ("dt", "tau0", "fs", "lambda"):
         assert sc[key] == scene[key], key
     assert (sc["line0"], sc["sample0"], sc["rows"], sc["cols"]) == (12.0, 34.0, 200, 300)
