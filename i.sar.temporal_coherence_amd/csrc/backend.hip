@@ -23,6 +23,7 @@
 #include <stdexcept>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -158,6 +159,38 @@ struct StageTimer {
     }
 };
 
+/* Scratch of the allocation-free entry point (tcoh_run_device_noalloc):
+   a file-scope device array, of which every GPU the library runs on holds
+   its own copy for the life of the process. Host code reaches a device's
+   copy only through the address hipGetSymbolAddress gives with that device
+   current (noalloc_base): the array's name is a host-side stand-in. */
+#define TCOH_NOALLOC_BYTES ((size_t)256 << 20)
+#define TCOH_NOALLOC_ALIGN 256
+#define TCOH_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[TCOH_NOALLOC_BYTES];
+
+namespace {
+std::mutex g_noalloc_mu;
+void *g_noalloc_base[TCOH_MAX_DEVICES];
+} // namespace
+
+/* The device address of the current device's scratch array (looked up
+   once per device, at the device's first ocl_open). */
+static void *noalloc_base(int device)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+
+    if (device < 0 || device >= TCOH_MAX_DEVICES)
+        fail("Device %d beyond the %d the library tracks", device, TCOH_MAX_DEVICES);
+    if (!g_noalloc_base[device]) {
+        void *p = nullptr;
+
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_noalloc_base[device] = p;
+    }
+    return g_noalloc_base[device];
+}
+
 /* What a captured strip depends on: a graph is replayed only for the same
    buffers, shape, stream and arithmetic. */
 struct GraphKey {
@@ -223,6 +256,8 @@ struct ocl {
     /* graph mode (tcoh_set_graph) and the captured strip */
     bool graph = false;
     struct GraphCache *gcache = nullptr;
+    /* this device's copy of the allocation-free entry point's scratch */
+    void *noalloc = nullptr;
 };
 
 static int env_precision()
@@ -319,6 +354,7 @@ extern "C" struct ocl *ocl_open(int platform, int device,
             o->precision = env_precision();
             o->timing = 0;
             check(hipSetDevice(device), "hipSetDevice");
+            o->noalloc = noalloc_base(device);
             check(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking),
                   "hipStreamCreate");
             check(hipGetDeviceProperties(&pr, device), "hipGetDeviceProperties");
@@ -697,6 +733,38 @@ extern "C" void tcoh_run_device_phases(struct ocl *o, const void *slc,
             fail("Linked phases requested without an output buffer");
         run_device(o, slc, amp, valid, padded_rows, rows, cols, coh, count,
                    pairs, phases, stream ? (hipStream_t)stream : o->stream);
+    });
+}
+
+extern "C" void tcoh_run_device_noalloc(struct ocl *o, const void *slc,
+                                        const void *amp, const void *valid,
+                                        int padded_rows, int rows, int cols,
+                                        void *coh, void *count, void *pairs,
+                                        void *stream)
+{
+    guarded_void([&] {
+        if (rows <= 0 || cols <= 0)
+            return;
+        const KParams kp = strip_params(o, padded_rows, rows, cols, pairs, nullptr);
+        const ScratchPlan pl = scratch_plan(kp, o->precision);
+        const size_t need = pl.total(TCOH_NOALLOC_ALIGN);
+
+        if (need > TCOH_NOALLOC_BYTES)
+            fail("Strip of %d x %d output pixels needs %zu bytes of scratch; the "
+                 "allocation-free entry point has %zu (smaller strips, or "
+                 "tcoh_run_device)", rows, cols, need, TCOH_NOALLOC_BYTES);
+        check(hipSetDevice(o->device), "hipSetDevice");
+        unsigned char *base = (unsigned char *)o->noalloc;
+        void *p[SC_COUNT];
+        size_t at = 0;
+
+        for (int k = 0; k < SC_COUNT; k++) {
+            p[k] = pl.bytes[k] ? base + at : nullptr;
+            at += (pl.bytes[k] + TCOH_NOALLOC_ALIGN - 1) / TCOH_NOALLOC_ALIGN *
+                  TCOH_NOALLOC_ALIGN;
+        }
+        enqueue_kernels(o, kp, pl, scratch_pointers(p), slc, amp, valid, coh, count,
+                        pairs, stream ? (hipStream_t)stream : o->stream, false);
     });
 }
 

@@ -344,6 +344,20 @@ void tcoh_run_phases(struct ocl *o, const float *slc, const float *amp,
                      int cols, float *coh, int *count, float *pairs,
                      float *phases);
 
+/* tcoh_run_device without any allocation: for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc). The strip's scratch (planar
+   amplitudes, SHP masks, split samples, EMI queues, the AD table) lives in
+   a fixed array of 256 MiB that every GPU the library runs on holds for the
+   process's life; strips needing more are rejected with an error (about
+   600 k output pixels at N=30 in the split forms). Calls on one GPU share
+   that array: calls that overlap on one GPU race. Not timed. No
+   counterpart in the reference. */
+void tcoh_run_device_noalloc(struct ocl *o, const void *slc, const void *amp,
+                             const void *valid, int padded_rows, int rows,
+                             int cols, void *coh, void *count, void *pairs,
+                             void *stream);
+
 /* Graph mode (default off): tcoh_run_device and tcoh_run_device_phases
    (and ocl_run) capture a strip's launches into a HIP graph on the first
    call with given buffers, shape, stream and precision, and replay it with
