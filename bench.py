@@ -35,6 +35,11 @@ import synth  # noqa: E402
 
 METRIC = "Mpixel/s at N=30, 21x7 KS+EVD: mixed-prec vs plain FP32 GPU, |dγ|<=1e-4"
 METRIC_EMI = "Mpixel/s at N=30, 21x7 AD+EMI, all pairs: mixed-prec vs plain FP32 GPU"
+METRIC_BATCH = ("aggregate Mpixel/s of six stacks N in {8,12,20,30,45,60}, 21x7 KS/tlog + "
+                "EVD, batched mixed-prec (one graph) vs per-stack sequential plain FP32 GPU")
+# The six stacks of the batched workload: depth, frame, SHP test.
+C4_STACKS = [(8, 2048, 3072, "ks"), (12, 2048, 2048, "tlog"), (20, 1536, 2048, "ks"),
+             (30, 1024, 1024, "tlog"), (45, 768, 768, "ks"), (60, 512, 512, "tlog")]
 METRIC_GEO = ("Mpixel/s at N=60, 21x7 KS+EVD, geometric phase removed: mixed-prec vs "
               "plain FP32 GPU, one GPU's 1024 x 25000 share")
 # HBM bytes per k_link launch (512 rows x 2048 px, split FP16, N=30) from
@@ -203,6 +208,225 @@ def run_graph(tcoh, s, pcols, bufs, rows, cols, precision, steps, torch, want):
     return (t1 - t0) * 1e3 / steps, same
 
 
+def link_bound(n, looks, est):
+    """k_link's algorithmic FLOPs of a strip's estimated pixels (looks:
+    their SHP counts) and its time at the bound of the split forms: the
+    three split products of the covariance on the matrix cores plus the
+    linking on the FP32 vector ALUs."""
+    flop_cov = float(np.sum(4.0 * n * (n + 1) * looks))
+    flop_link = est * flops_eigen(n)
+    return flop_cov + flop_link, 3.0 * flop_cov / PEAK_F16_MFMA + flop_link / PEAK_FP32
+
+
+def mfma_executed(n, looks):
+    """Matrix-core FLOPs the split forms execute for these pixels: per
+    pixel ceil(L/32) chunks of 32 candidates, 12 v_mfma_f32_16x16x32 per
+    upper 16 x 16 date block pair."""
+    nbk = (16 if n <= 16 else (32 if n <= 32 else 64)) // 16
+    return float(np.sum(np.ceil(looks / 32.0))) * 12 * nbk * (nbk + 1) // 2 * 16 * 16 * 32 * 2
+
+
+def timed_steps(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def config4(args, tcoh, torch, cref, fp64):
+    """The batched workload: six independent stacks (C4_STACKS; the
+    quadrant scene of gamma0 {0.9, 0.6, 0.3, 0.1}, tau 48 d, seed 100 + k
+    for stack k), 21x7 windows, alpha 0.05, min_shp 20, EVD, no pair maps.
+    One step = every stack once. Timed: the batch (tcoh_batch_run) in the
+    split FP16 form replayed as one captured graph; beside it the same
+    batch eagerly, the stacks one after the other in the same form, and the
+    per-stack sequential plain FP32 path the metric compares with."""
+    wa, wr = 21, 7
+    ha, hr = wa // 2, wr // 2
+    st = torch.cuda.Stream()
+    stacks = []
+    t0 = time.perf_counter()
+    for k, (n, rows, cols, test) in enumerate(C4_STACKS):
+        s_opts = dict(window=(wa, wr), shp_test=test, alpha=0.05, min_shp=20,
+                      estimator="evd", bias=False, pairs=None)
+        s = tcoh.make_settings(n, **s_opts)
+        reg, params = synth.quadrant_regions(rows, cols)
+        stack = synth.scene_stack_torch(n, rows, cols, reg, params, seed=100 + k)
+        di, dq = stack.real.contiguous(), stack.imag.contiguous()
+        del stack
+        slc, amp, valid, _ = prepare_on_device(tcoh, torch, s, di, dq, ha, hr)
+        del di, dq
+        stacks.append(dict(
+            n=n, rows=rows, cols=cols, test=test, s=s, s_opts=s_opts, pcols=cols + 2 * hr,
+            slc=slc, amp=amp, valid=valid,
+            coh=torch.empty((rows, cols), dtype=torch.float32, device="cuda"),
+            count=torch.empty((rows, cols), dtype=torch.int32, device="cuda"),
+            pairs=torch.empty((1, rows, cols), dtype=torch.float32, device="cuda")))
+    torch.cuda.synchronize()
+    total_px = sum(c["rows"] * c["cols"] for c in stacks)
+    log("six stacks (%d px) ready in %.1f s" % (total_px, time.perf_counter() - t0))
+
+    def args_of(c):
+        return (c["slc"].data_ptr(), c["amp"].data_ptr(), c["valid"].data_ptr(),
+                c["slc"].shape[0], c["rows"], c["cols"], c["coh"].data_ptr(),
+                c["count"].data_ptr(), c["pairs"].data_ptr())
+
+    def snapshot():
+        return [(c["coh"].cpu().numpy().view(np.uint32).copy(), c["count"].cpu().numpy())
+                for c in stacks]
+
+    def same_maps(a, b):
+        return all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+                   for x, y in zip(a, b))
+
+    out_forms = {}
+    per_stack = []
+    maps = {}
+    for form in (args.precision, "plain"):
+        bes = [tcoh.Backend(c["s"], c["pcols"], platform=0, device=0, precision=form)
+               for c in stacks]
+
+        def sequential():
+            for be, c in zip(bes, stacks):
+                be.run_device(*args_of(c), stream=st.cuda_stream)
+        steps = args.steps if form == args.precision else max(2, args.steps // 2)
+        seq_ms = timed_steps(sequential, steps, args.warmup, torch)
+        maps[form] = snapshot()
+        res = {"sequential_ms": seq_ms, "sequential_value": total_px / (seq_ms * 1e-3) / 1e6,
+               "steps": steps}
+        if form == args.precision:
+            # per-stack stage times (HIP events on the run's stream), alone
+            for be, c in zip(bes, stacks):
+                be.set_timing(True)
+                be.run_device(*args_of(c), stream=st.cuda_stream)
+                ms, nl = be.stage_times()
+                be.set_timing(False)
+                cnt = maps[form][len(per_stack)][1]
+                looks = cnt[cnt >= c["s"].min_looks].astype(np.float64)
+                flop, tb = link_bound(c["n"], looks, looks.size)
+                per_stack.append(dict(
+                    n=c["n"], frame=[c["rows"], c["cols"]], shp_test=c["test"],
+                    tiles=list(be.get_tiles()), estimated_pixels=int(looks.size),
+                    mean_shp=float(looks.mean()), shp_range=[int(cnt[cnt >= 0].min()),
+                                                            int(cnt.max())],
+                    stage_ms=dict(zip(["prep", "shp", "link"], np.round(ms[:3], 3).tolist())),
+                    link_launches=int(nl[2]), link_flop=flop, link_bound_ms=tb * 1e3,
+                    mfma_flop=mfma_executed(c["n"], looks)))
+            with tcoh.Batch() as batch:
+                items = [(be,) + args_of(c) for be, c in zip(bes, stacks)]
+                res["batch_eager_ms"] = timed_steps(
+                    lambda: batch.run(items, stream=st.cuda_stream), steps, args.warmup, torch)
+                order, lane = batch.plan()
+                res["schedule"] = {"order_n": [stacks[i]["n"] for i in order],
+                                   "lane_of_n": {str(stacks[i]["n"]): lane[i]
+                                                 for i in range(len(stacks))},
+                                   "streams": 4}
+            same_eager = same_maps(snapshot(), maps[form])
+            with tcoh.Batch(graph=True) as batch:
+                items = [(be,) + args_of(c) for be, c in zip(bes, stacks)]
+                res["batch_graph_ms"] = timed_steps(
+                    lambda: batch.run(items, stream=st.cuda_stream), steps,
+                    max(2, args.warmup), torch)
+            got = snapshot()
+            res["batch_maps_equal_sequential"] = bool(same_eager and same_maps(got, maps[form]))
+            log("%s: sequential %.2f ms, batch eager %.2f ms, batch graph %.2f ms, maps "
+                "equal: %s" % (form, seq_ms, res["batch_eager_ms"], res["batch_graph_ms"],
+                               res["batch_maps_equal_sequential"]))
+        else:
+            log("%s: sequential %.2f ms" % (form, seq_ms))
+        for be in bes:
+            be.close()
+        out_forms[form] = res
+
+    mix = out_forms[args.precision]
+    ms = mix["batch_graph_ms"]
+    plain_ms = out_forms["plain"]["sequential_ms"]
+    flop = sum(p["link_flop"] for p in per_stack)
+    link_s = sum(p["stage_ms"]["link"] for p in per_stack) * 1e-3
+    launches = sum(p["link_launches"] for p in per_stack)
+    bound_s = sum(p["link_bound_ms"] for p in per_stack) * 1e-3
+    mfma = sum(p["mfma_flop"] for p in per_stack)
+    roofline = {"bound": "mfma", "achieved": flop / link_s / 1e12, "peak": flop / bound_s / 1e12,
+                "unit": "TFLOP/s", "frac": bound_s / link_s, "traffic": None,
+                "kernel": "k_link %s over the six stacks (split-16-bit MFMA covariance + FP32 "
+                          "linking), each stack's launches timed alone" % args.precision,
+                "avg_launch_ms": link_s * 1e3 / max(1, launches),
+                "launches_per_step": launches,
+                "mfma": {"executed_flop_per_step": mfma,
+                         "util_of_2.5PF_over_k_link": mfma / link_s / PEAK_F16_MFMA}}
+    out = {
+        "metric": METRIC_BATCH, "value": total_px / (ms * 1e-3) / 1e6, "unit": "Mpixel/s",
+        "n_gpus": 1, "steps": mix["steps"], "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 hi/lo split, f32 accumulate" if args.precision == "f16"
+                 else "bf16 hi/lo split, f32 accumulate",
+        "data": "synthetic",
+        "config": {"workload": "six stacks N=8 2048x3072 ks, N=12 2048x2048 tlog, N=20 "
+                               "1536x2048 ks, N=30 1024x1024 tlog, N=45 768x768 ks, N=60 "
+                               "512x512 tlog; quadrant scene gamma0 {0.9,0.6,0.3,0.1}, tau 48 d, "
+                               "seeds 100..105; 21x7 window, alpha 0.05, min_shp 20, EVD",
+                   "pixels_per_step": total_px, "stacks": per_stack},
+        "batch": dict(mix, path="tcoh_batch_run in graph mode: the six strips forked onto 4 "
+                                "streams in a longest-first schedule, captured as one graph"),
+        "plain_sequential": dict(out_forms["plain"],
+                                 value=total_px / (plain_ms * 1e-3) / 1e6),
+        "speedup_vs_plain": plain_ms / ms,
+        "roofline": roofline,
+    }
+    if args.crop > 0:
+        crop = min(args.crop, 64)
+        acc = {}
+        t0 = time.perf_counter()
+        for c, (coh_m, cnt_m) in zip(stacks, maps[args.precision]):
+            rows, cols, n = c["rows"], c["cols"], c["n"]
+            y0, x0 = rows // 2 - crop // 2, cols // 2 - crop // 2
+            bc, bn, bp, bph = band_maps(tcoh, torch, c["s_opts"], n, c["slc"], c["amp"],
+                                        c["valid"], c["pcols"], y0, crop, cols,
+                                        args.precision)
+            gm = {args.precision: (coh_m.view(np.float32).reshape(rows, cols),
+                                   cnt_m.reshape(rows, cols), np.zeros((0, rows, cols)))}
+            a = crop_accuracy(tcoh, cref, fp64, c["slc"].cpu().numpy(), c["amp"].cpu().numpy(),
+                              c["valid"].cpu().numpy(), c["pcols"], n, c["s_opts"], y0, x0,
+                              crop, gm, {args.precision: (M2C_BAND[args.precision], bp, bph)})
+            f = a[args.precision]
+            acc["N=%d" % n] = {k: f[k] for k in (
+                "max_abs_dgamma", "rms_dgamma", "pixels_over_1e-4", "gate_1e-4",
+                "max_abs_dphase_rad", "rms_dphase_rad", "shp_count_mismatches",
+                "nan_mask_equal")}
+            acc["N=%d" % n]["band_rerun_equals_timed_run"] = bool(
+                np.array_equal(bc.view(np.uint32), coh_m.reshape(rows, cols)[y0:y0 + crop]) and
+                np.array_equal(bn, cnt_m.reshape(rows, cols)[y0:y0 + crop]))
+        acc["crop"] = "%d x %d at each frame's centre" % (crop, crop)
+        acc["gate_1e-4"] = all(v["gate_1e-4"] for k, v in acc.items() if k.startswith("N="))
+        out["accuracy"] = acc
+        log("accuracy crops in %.1f s" % (time.perf_counter() - t0))
+    if not args.no_cpu:
+        threads = cpu_threads()
+        band = args.cpu_rows or 32
+        est_s, sample = 0.0, []
+        for c in stacks:
+            so = cref.settings(c["n"], **c["s_opts"])
+            pr_b = band + wa - 1
+            t0 = time.perf_counter()
+            cref.run(so, c["pcols"], c["slc"][:pr_b].cpu().numpy(), c["amp"][:pr_b].cpu().numpy(),
+                     c["valid"][:pr_b].cpu().numpy(), pr_b, band, c["cols"], with_mask=False,
+                     nthreads=threads)
+            dt = time.perf_counter() - t0
+            est_s += dt * c["rows"] / band
+            sample.append("N=%d %d x %d px in %.2f s" % (c["n"], band, c["cols"], dt))
+        out["cpu_baseline"] = {
+            "value": total_px / est_s / 1e6, "unit": "Mpixel/s", "cores": threads,
+            "kind": "port", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": "the first %d output rows of each stack (%s), FP32 C restatement of "
+                      "the reference kernel, OpenMP over rows; the batch's CPU time is each "
+                      "stack's band time scaled to its rows" % (band, "; ".join(sample))}
+    print(json.dumps(out))
+
+
 # Bias-threshold band of each form's m2c rounding (as tests/test_gpu_parity.py
 # derives it): the oracle takes the form's own decision inside it.
 M2C_BAND = {"plain": 1e-6, "f16": 1e-6, "bf16": 1e-5, "emi": 1e-10}
@@ -339,10 +563,11 @@ def main():
                     help="skip the graph-replay rate (tcoh_set_graph)")
     ap.add_argument("--no-region", action="store_true",
                     help="skip the host-to-host rate through tcoh_run_region")
-    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 5],
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 4, 5],
                     help="2: N=30 KS+EVD, bias, consecutive pairs, gamma0 ~ U(0.05, "
                          "0.95) (the headline); 3: N=30 AD+EMI, bias, all 435 pairs, "
-                         "gamma0 ~ U(0.6, 0.99); 5: N=60 KS+EVD, bias, flat-earth and "
+                         "gamma0 ~ U(0.6, 0.99); 4: six stacks N in {8, 12, 20, 30, 45, "
+                         "60} batched (KS/tlog + EVD); 5: N=60 KS+EVD, bias, flat-earth and "
                          "topographic phase removed at DEM heights, one GPU's share "
                          "(1024 x 25000) of 8192 x 25000")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -360,6 +585,9 @@ def main():
     from oracle import cref, fp64
 
     tcoh = ge.load_package()
+    if args.config == 4:
+        config4(args, tcoh, torch, cref, fp64)
+        return
     c5 = args.config == 5
     n = 60 if c5 else 30
     rows = args.rows or (1024 if c5 else 2048)

@@ -11,5 +11,5 @@ The directory name is not a Python identifier; load it by path, e.g.
 """
 
 from . import api  # noqa: F401
-from .api import (Backend, make_settings, pair_list, prepare_strip,  # noqa: F401
+from .api import (Backend, Batch, make_settings, pair_list, prepare_strip,  # noqa: F401
                   temporal_coherence)

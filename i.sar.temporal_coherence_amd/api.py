@@ -9,6 +9,8 @@ the same entry points of the HIP backend:
   Backend                   one ocl_open handle; run() = ocl_run on host
                             arrays, run_device() = tcoh_run_device on
                             device pointers (e.g. torch tensors)
+  Batch                     several handles' strips (stacks of any depth)
+                            as one unit of work (tcoh_batch_run)
   temporal_coherence        the module's strip loop (main.c:718-825) over a
                             planar complex stack, with read_strip's
                             preparation (main.c:52-127): returns the
@@ -144,6 +146,17 @@ def lib():
     L.tcoh_set_timing.argtypes = [vp, C.c_int]
     L.tcoh_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
     L.tcoh_stage_launches.argtypes = [vp, C.POINTER(C.c_int)]
+    L.tcoh_set_tiles.argtypes = [vp, vp]
+    L.tcoh_get_tiles.argtypes = [vp, vp]
+    L.tcoh_batch_create.argtypes = [C.c_int]
+    L.tcoh_batch_create.restype = vp
+    L.tcoh_batch_set_graph.argtypes = [vp, C.c_int]
+    L.tcoh_batch_run.argtypes = [vp, vp, C.c_int, vp]
+    L.tcoh_batch_plan.argtypes = [vp, vp, vp]
+    L.tcoh_batch_destroy.argtypes = [vp]
+    L.tcoh_strip_cost.argtypes = [C.POINTER(Settings), C.c_int, C.c_int]
+    L.tcoh_strip_cost.restype = C.c_double
+    L.tcoh_batch_schedule.argtypes = [C.c_int, vp, C.c_int, vp, vp]
     L.tcoh_set_error_mode.argtypes = [C.c_int]
     L.tcoh_last_error.restype = C.c_char_p
     L.tcoh_prepare_strip.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
@@ -201,6 +214,9 @@ def lib():
     L.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
     L.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
     L.hipDeviceSynchronize.argtypes = []
+    L.hipStreamCreateWithFlags.argtypes = [C.POINTER(vp), C.c_uint]
+    L.hipStreamSynchronize.argtypes = [vp]
+    L.hipStreamDestroy.argtypes = [vp]
     L.tcoh_set_error_mode(1)
     _lib = L
     return L
@@ -276,6 +292,18 @@ class Backend:
 
     def set_timing(self, on):
         lib().tcoh_set_timing(self.h, 1 if on else 0)
+
+    def set_tiles(self, tiles):
+        """Launch shapes (shp_stripe, link_stripe, band_px), or None for the
+        tuned table's (tcoh_set_tiles)."""
+        t = None if tiles is None else (C.c_int * 3)(*tiles)
+        lib().tcoh_set_tiles(self.h, t)
+        _raise_if_error()
+
+    def get_tiles(self):
+        t = (C.c_int * 3)()
+        lib().tcoh_get_tiles(self.h, t)
+        return tuple(t)
 
     def set_graph(self, on):
         """Graph mode: strips replayed as one captured HIP graph."""
@@ -366,6 +394,81 @@ class Backend:
     def close(self):
         if self.h:
             lib().ocl_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("o", C.c_void_p), ("slc", C.c_void_p), ("amp", C.c_void_p),
+                ("valid", C.c_void_p), ("padded_rows", C.c_int), ("rows", C.c_int),
+                ("cols", C.c_int), ("coh", C.c_void_p), ("count", C.c_void_p),
+                ("pairs", C.c_void_p)]
+
+
+def strip_cost(settings, rows, cols):
+    """tcoh_strip_cost: the schedule's estimate of a strip's work."""
+    return lib().tcoh_strip_cost(C.byref(settings), rows, cols)
+
+
+def batch_schedule(costs, streams):
+    """tcoh_batch_schedule: (order, lane) of the longest-first list
+    schedule of the costs over `streams` lanes."""
+    n = len(costs)
+    c = (C.c_double * max(1, n))(*costs)
+    order, lane = (C.c_int * max(1, n))(), (C.c_int * max(1, n))()
+    lib().tcoh_batch_schedule(n, c, streams, order, lane)
+    return list(order)[:n], list(lane)[:n]
+
+
+class Batch:
+    """A tcoh_batch: strips of several handles run as one unit of work.
+
+    run() takes items (backend, slc_ptr, amp_ptr, valid_ptr, padded_rows,
+    rows, cols, coh_ptr, count_ptr, pairs_ptr) with device pointers (ints)
+    and returns without waiting."""
+
+    def __init__(self, streams=0, graph=False):
+        self.h = lib().tcoh_batch_create(streams)
+        _raise_if_error()
+        if not self.h:
+            raise RuntimeError("tcoh_batch_create failed")
+        self.n = 0
+        if graph:
+            self.set_graph(True)
+
+    def set_graph(self, on):
+        lib().tcoh_batch_set_graph(self.h, 1 if on else 0)
+        _raise_if_error()
+
+    def run(self, items, stream=None):
+        arr = (BatchItem * max(1, len(items)))()
+        for k, (be, slc, amp, valid, prow, rows, cols, coh, count, pairs) in enumerate(items):
+            arr[k] = BatchItem(be.h if hasattr(be, "h") else be, slc, amp, valid, prow, rows,
+                               cols, coh, count, pairs)
+        lib().tcoh_batch_run(self.h, arr, len(items), stream)
+        _raise_if_error()
+        self.n = len(items)
+
+    def plan(self):
+        """The last run's (order, lane)."""
+        order, lane = (C.c_int * max(1, self.n))(), (C.c_int * max(1, self.n))()
+        lib().tcoh_batch_plan(self.h, order, lane)
+        return list(order)[:self.n], list(lane)[:self.n]
+
+    def close(self):
+        if self.h:
+            lib().tcoh_batch_destroy(self.h)
             self.h = None
 
     def __enter__(self):
@@ -524,6 +627,26 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class DeviceStream:
+    """A non-blocking hipStream_t of the library's runtime (one that graph
+    capture accepts); ptr is the stream handle."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        if lib().hipStreamCreateWithFlags(C.byref(p), 1) != 0:
+            raise RuntimeError("hipStreamCreateWithFlags failed")
+        self.ptr = p.value
+
+    def synchronize(self):
+        if lib().hipStreamSynchronize(self.ptr) != 0:
+            raise RuntimeError("hipStreamSynchronize failed")
+
+    def destroy(self):
+        if self.ptr:
+            lib().hipStreamDestroy(self.ptr)
+            self.ptr = None
 
 
 def make_geometry(phase, heights, h0=0.0, step=16):

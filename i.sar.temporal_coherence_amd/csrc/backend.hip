@@ -11,7 +11,8 @@
  *             coherences, phase linking and temporal coherence
  * Buffers only grow (ocl.c:264-277). The reference splits its launches
  * into ~0.5 s chunks against GPU watchdogs (ocl.c:329-363); here bands of
- * about a million pixels bound every launch instead.
+ * about a million pixels (the tuned band size, tiles.h) bound every launch
+ * instead.
  */
 
 #include <math.h>
@@ -28,6 +29,7 @@
 #include <vector>
 
 #include "common.h"
+#include "tiles.h"
 #include "../../include/tcoh_hip.h"
 
 namespace {
@@ -197,13 +199,15 @@ struct GraphKey {
     const void *slc, *amp, *valid, *coh, *count, *pairs, *phase;
     int padded_rows, rows, cols, precision;
     hipStream_t stream;
+    int tiles[3];
 
     bool operator==(const GraphKey &k) const
     {
         return slc == k.slc && amp == k.amp && valid == k.valid && coh == k.coh &&
                count == k.count && pairs == k.pairs && phase == k.phase &&
                padded_rows == k.padded_rows && rows == k.rows && cols == k.cols &&
-               precision == k.precision && stream == k.stream;
+               precision == k.precision && stream == k.stream &&
+               !memcmp(tiles, k.tiles, sizeof(tiles));
     }
 };
 
@@ -231,6 +235,10 @@ struct ocl {
     struct settings s;
     int padded_cols;
     int precision;
+    /* launch shapes: SHP and linking stripe widths, pixels per linking
+       launch (tiles.h), and whether tcoh_set_tiles fixed them */
+    int tiles[3];
+    bool tiles_set;
     int timing;
     double stage_ms[4];
     int stage_launches[4];
@@ -271,6 +279,32 @@ static int env_precision()
     if (!strcmp(e, "bf16"))
         return TCOH_MIXED_BF16;
     return TCOH_MIXED_F16;
+}
+
+/* The tuned shapes of a handle's estimator, form and depth (tiles.h):
+   the nearest tuned depth with the same lanes per pixel, else the nearest
+   one, else the built-in shapes. */
+static void tile_lookup(struct ocl *o)
+{
+    const int n = o->s.ndates, emi = o->s.emi ? 1 : 0;
+    const int mixed = o->precision != TCOH_PLAIN && !emi;
+    auto bucket = [](int d) { return d <= 16 ? 0 : (d <= 32 ? 1 : 2); };
+    const TileEntry *best = nullptr;
+    long best_d = 0;
+
+    for (const TileEntry &e : TILE_TABLE) {
+        if (e.n <= 0 || e.emi != emi || e.mixed != mixed)
+            continue;
+        const long d = (bucket(e.n) != bucket(n) ? 1000L : 0L) + std::abs(e.n - n);
+
+        if (!best || d < best_d) {
+            best = &e;
+            best_d = d;
+        }
+    }
+    o->tiles[0] = best ? best->shp_stripe : TILE_STRIPE_DEFAULT;
+    o->tiles[1] = best ? best->link_stripe : TILE_STRIPE_DEFAULT;
+    o->tiles[2] = best && best->band_px > 0 ? best->band_px : TILE_BAND_PX;
 }
 
 static bool eig_supported(int n);
@@ -352,6 +386,8 @@ extern "C" struct ocl *ocl_open(int platform, int device,
             o->s = *s;
             o->padded_cols = padded_cols;
             o->precision = env_precision();
+            o->tiles_set = false;
+            tile_lookup(o);
             o->timing = 0;
             check(hipSetDevice(device), "hipSetDevice");
             o->noalloc = noalloc_base(device);
@@ -384,10 +420,36 @@ extern "C" void tcoh_set_precision(struct ocl *o, int precision)
         if (precision < TCOH_PLAIN || precision > TCOH_MIXED_BF16)
             fail("Unknown precision %d", precision);
         o->precision = precision;
+        if (!o->tiles_set)
+            tile_lookup(o);
     });
 }
 
 extern "C" void tcoh_set_timing(struct ocl *o, int on) { o->timing = on; }
+
+extern "C" void tcoh_set_tiles(struct ocl *o, const int tiles[3])
+{
+    guarded_void([&] {
+        if (!tiles) {
+            o->tiles_set = false;
+            tile_lookup(o);
+            return;
+        }
+        if (tiles[0] < 0 || tiles[1] < 0 || tiles[2] < 0)
+            fail("Launch shapes must not be negative (%d, %d, %d)", tiles[0], tiles[1],
+                 tiles[2]);
+        o->tiles[0] = tiles[0];
+        o->tiles[1] = tiles[1];
+        o->tiles[2] = tiles[2] > 0 ? tiles[2] : TILE_BAND_PX;
+        o->tiles_set = true;
+    });
+}
+
+extern "C" void tcoh_get_tiles(const struct ocl *o, int tiles[3])
+{
+    for (int k = 0; k < 3; k++)
+        tiles[k] = o->tiles[k];
+}
 
 extern "C" void tcoh_stage_times(const struct ocl *o, double ms[4])
 {
@@ -442,6 +504,13 @@ struct ScratchPlan {
     }
 };
 
+/* Pixels of one linking group: 64 / (lanes per pixel), the lanes being 8
+   up to 16 dates, 16 up to 32 and 64 beyond (k_link.hip's LinkShape). */
+static int link_group_pixels(int n)
+{
+    return n <= 16 ? 8 : (n <= 32 ? 4 : 1);
+}
+
 /* The strip's settings as the kernels take them, after the strip checks. */
 static KParams strip_params(const struct ocl *o, int padded_rows, int rows, int cols,
                             const void *pairs, void *phase)
@@ -480,6 +549,14 @@ static KParams strip_params(const struct ocl *o, int padded_rows, int rows, int 
     kp.ad_crit = kernel_float(s.ad_crit);
     kp.tlog_crit = kernel_float(s.tlog_crit);
     kp.phase = (float *)phase;
+    /* stripes per row -> stripe width in tiles (SHP tiles: 64 pixels;
+       linking tiles: a group's pixels) */
+    kp.shp_stripe = o->tiles[0] > 0 ? ((cols + 63) / 64 + o->tiles[0] - 1) / o->tiles[0] : 0;
+    {
+        const int ppw = link_group_pixels(s.ndates), nbx = (cols + ppw - 1) / ppw;
+
+        kp.link_stripe = o->tiles[1] > 0 ? (nbx + o->tiles[1] - 1) / o->tiles[1] : 0;
+    }
     if ((size_t)kp.npad * shp_bucket(kp.n) > (size_t)INT32_MAX)
         fail("Strip of %d pixels too large for %d dates (at most %d pixels; "
              "use strips within ocl_open's max_alloc)",
@@ -487,7 +564,7 @@ static KParams strip_params(const struct ocl *o, int padded_rows, int rows, int 
     return kp;
 }
 
-static ScratchPlan scratch_plan(const KParams &kp, int precision)
+static ScratchPlan scratch_plan(const KParams &kp, int precision, int band_px)
 {
     ScratchPlan pl;
     const size_t npad = (size_t)kp.npad, nout = (size_t)kp.rows * kp.cols;
@@ -498,7 +575,7 @@ static ScratchPlan scratch_plan(const KParams &kp, int precision)
     /* EMI takes the plain form with FP64 sums whatever the setting */
     pl.mixed = precision != TCOH_PLAIN && !kp.emi;
     pl.bf16 = precision == TCOH_MIXED_BF16;
-    pl.chunk = std::max(1, (1 << 20) / kp.cols);
+    pl.chunk = std::max(1, band_px / kp.cols);
     pl.cap = std::min(pl.chunk, kp.rows) * kp.cols;
     pl.bytes[SC_AMPT] = npad * nb * sizeof(float);
     pl.bytes[SC_MASK] = nout * kp.words * sizeof(uint32_t);
@@ -635,9 +712,9 @@ static void enqueue_kernels(struct ocl *o, const KParams &kp, const ScratchPlan 
 }
 
 /* One strip's launches on st, in the handle's scratch (allocated here
-   when it has to grow). With capture set the stream is being captured
-   into a graph: the eager run of the same call has allocated everything,
-   and nothing is timed. */
+   when it has to grow). With capture set nothing is timed: the stream is
+   being captured into a graph (the eager run of the same call has
+   allocated everything), or the strip is one of a batch. */
 static void enqueue_strip(struct ocl *o, const void *slc, const void *amp,
                           const void *valid, int padded_rows, int rows, int cols,
                           void *coh, void *count, void *pairs, void *phase, hipStream_t st,
@@ -648,7 +725,7 @@ static void enqueue_strip(struct ocl *o, const void *slc, const void *amp,
     const KParams kp = strip_params(o, padded_rows, rows, cols, pairs, phase);
 
     check(hipSetDevice(o->device), "hipSetDevice");
-    const ScratchPlan pl = scratch_plan(kp, o->precision);
+    const ScratchPlan pl = scratch_plan(kp, o->precision, o->tiles[2]);
     const StripScratch sc = handle_scratch(o, pl);
 
     enqueue_kernels(o, kp, pl, sc, slc, amp, valid, coh, count, pairs, st,
@@ -665,8 +742,9 @@ static void run_device(struct ocl *o, const void *slc, const void *amp,
                        void *coh, void *count, void *pairs, void *phase, hipStream_t st)
 {
     GraphCache &gc = *o->gcache;
-    const GraphKey key = {slc,         amp,  valid, coh,  count,        pairs, phase,
-                          padded_rows, rows, cols,  o->precision, st};
+    const GraphKey key = {slc,  amp,  valid,        coh, count,
+                          pairs, phase, padded_rows, rows, cols,
+                          o->precision, st, {o->tiles[0], o->tiles[1], o->tiles[2]}};
 
     if (!o->graph || o->timing) {
         enqueue_strip(o, slc, amp, valid, padded_rows, rows, cols, coh, count, pairs, phase,
@@ -746,7 +824,7 @@ extern "C" void tcoh_run_device_noalloc(struct ocl *o, const void *slc,
         if (rows <= 0 || cols <= 0)
             return;
         const KParams kp = strip_params(o, padded_rows, rows, cols, pairs, nullptr);
-        const ScratchPlan pl = scratch_plan(kp, o->precision);
+        const ScratchPlan pl = scratch_plan(kp, o->precision, o->tiles[2]);
         const size_t need = pl.total(TCOH_NOALLOC_ALIGN);
 
         if (need > TCOH_NOALLOC_BYTES)
@@ -766,6 +844,230 @@ extern "C" void tcoh_run_device_noalloc(struct ocl *o, const void *slc,
         enqueue_kernels(o, kp, pl, scratch_pointers(p), slc, amp, valid, coh, count,
                         pairs, stream ? (hipStream_t)stream : o->stream, false);
     });
+}
+
+/* Batches: the strips of several handles (stacks of any depth, settings
+   and form) on one device, issued as one unit of work on a caller's
+   stream. The strips run concurrently on the batch's own streams, which
+   fork from and join back into the caller's stream through events, in a
+   longest-first list schedule of their estimated work. In graph mode the
+   whole batch (every stage of every strip, the fork and the join) is
+   captured once and replayed as one graph. */
+struct tcoh_batch {
+    int device = -1;
+    int nstreams = 0;
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    std::vector<int> order, lane; /* the last schedule */
+    bool graph = false;
+    /* the captured batch and what it depends on */
+    bool valid = false;
+    std::vector<tcoh_batch_item> key;
+    std::vector<int> key_state; /* per item: precision and launch shapes */
+    hipStream_t key_stream = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void release_graph()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (g)
+            (void)hipGraphDestroy(g);
+        exec = nullptr;
+        g = nullptr;
+        valid = false;
+    }
+};
+
+extern "C" double tcoh_strip_cost(const struct settings *s, int rows, int cols)
+{
+    const double n = s->ndates, w = (double)s->win_az * s->win_rg;
+
+    return (double)rows * cols * (w * n * n + n * n * n);
+}
+
+extern "C" void tcoh_batch_schedule(int n_items, const double *cost, int streams,
+                                    int *order, int *lane)
+{
+    std::vector<int> idx(n_items > 0 ? n_items : 0);
+    std::vector<double> load(streams > 0 ? streams : 1, 0.0);
+
+    for (int i = 0; i < n_items; i++)
+        idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return cost[a] > cost[b]; });
+    for (int k = 0; k < n_items; k++) {
+        const int i = idx[k];
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+
+        order[k] = i;
+        lane[i] = l;
+        load[l] += cost[i];
+    }
+}
+
+extern "C" struct tcoh_batch *tcoh_batch_create(int streams)
+{
+    return guarded(
+        [&]() -> struct tcoh_batch * {
+            if (streams < 0 || streams > 64)
+                fail("A batch takes 1 to 64 streams, not %d", streams);
+            struct tcoh_batch *b = new struct tcoh_batch();
+
+            b->nstreams = streams ? streams : 4;
+            return b;
+        },
+        nullptr);
+}
+
+extern "C" void tcoh_batch_set_graph(struct tcoh_batch *b, int on)
+{
+    guarded_void([&] {
+        b->graph = on != 0;
+        if (!b->graph)
+            b->release_graph();
+    });
+}
+
+extern "C" void tcoh_batch_plan(const struct tcoh_batch *b, int *order, int *lane)
+{
+    for (size_t k = 0; k < b->order.size(); k++) {
+        order[k] = b->order[k];
+        lane[k] = b->lane[k];
+    }
+}
+
+/* Fork, every strip on its lane's stream in schedule order, join. */
+static void batch_enqueue(struct tcoh_batch *b, const struct tcoh_batch_item *it,
+                          int n, hipStream_t st)
+{
+    std::vector<char> used(b->nstreams, 0);
+
+    for (int i = 0; i < n; i++)
+        used[b->lane[i]] = 1;
+    check(hipEventRecord(b->fork, st), "hipEventRecord(batch fork)");
+    for (int l = 0; l < b->nstreams; l++)
+        if (used[l])
+            check(hipStreamWaitEvent(b->streams[l], b->fork, 0),
+                  "hipStreamWaitEvent(batch fork)");
+    for (int k = 0; k < n; k++) {
+        const struct tcoh_batch_item &x = it[b->order[k]];
+
+        enqueue_strip(x.o, x.slc, x.amp, x.valid, x.padded_rows, x.rows, x.cols, x.coh,
+                      x.count, x.pairs, nullptr, b->streams[b->lane[b->order[k]]], true);
+    }
+    for (int l = 0; l < b->nstreams; l++)
+        if (used[l]) {
+            check(hipEventRecord(b->join[l], b->streams[l]), "hipEventRecord(batch join)");
+            check(hipStreamWaitEvent(st, b->join[l], 0), "hipStreamWaitEvent(batch join)");
+        }
+}
+
+extern "C" void tcoh_batch_run(struct tcoh_batch *b, const struct tcoh_batch_item *items,
+                               int n_items, void *stream)
+{
+    guarded_void([&] {
+        if (n_items <= 0)
+            return;
+        if (!items)
+            fail("Batch of %d strips without items", n_items);
+        const int device = items[0].o ? items[0].o->device : -1;
+        std::vector<double> cost(n_items);
+        std::vector<int> state;
+
+        for (int i = 0; i < n_items; i++) {
+            const struct ocl *o = items[i].o;
+
+            if (!o)
+                fail("Batch item %d has no handle", i);
+            if (o->device != device)
+                fail("Batch item %d is on device %d, item 0 on device %d", i, o->device,
+                     device);
+            for (int j = 0; j < i; j++)
+                if (items[j].o == o)
+                    fail("Batch items %d and %d share a handle (each strip needs its "
+                         "own handle's scratch)", j, i);
+            cost[i] = tcoh_strip_cost(&o->s, items[i].rows, items[i].cols);
+            state.insert(state.end(), {o->precision, o->tiles[0], o->tiles[1], o->tiles[2]});
+        }
+        check(hipSetDevice(device), "hipSetDevice");
+        if (b->device < 0) {
+            b->device = device;
+            b->streams.resize(b->nstreams);
+            b->join.resize(b->nstreams);
+            check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming),
+                  "hipEventCreate(batch)");
+            for (int l = 0; l < b->nstreams; l++) {
+                check(hipStreamCreateWithFlags(&b->streams[l], hipStreamNonBlocking),
+                      "hipStreamCreate(batch)");
+                check(hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming),
+                      "hipEventCreate(batch)");
+            }
+        }
+        else if (b->device != device)
+            fail("Batch created on device %d given strips on device %d", b->device, device);
+        b->order.assign(n_items, 0);
+        b->lane.assign(n_items, 0);
+        tcoh_batch_schedule(n_items, cost.data(), b->nstreams, b->order.data(),
+                            b->lane.data());
+        hipStream_t st = stream ? (hipStream_t)stream : items[0].o->stream;
+
+        if (!b->graph) {
+            batch_enqueue(b, items, n_items, st);
+            return;
+        }
+        if (b->valid && b->key_stream == st && (int)b->key.size() == n_items &&
+            !memcmp(b->key.data(), items, n_items * sizeof(*items)) && b->key_state == state) {
+            check(hipGraphLaunch(b->exec, st), "hipGraphLaunch(batch)");
+            return;
+        }
+        /* eagerly first (the handles' scratch grows here), then captured */
+        batch_enqueue(b, items, n_items, st);
+        b->release_graph();
+        check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed),
+              "hipStreamBeginCapture(batch)");
+        try {
+            batch_enqueue(b, items, n_items, st);
+        }
+        catch (...) {
+            hipGraph_t g = nullptr;
+
+            (void)hipStreamEndCapture(st, &g);
+            if (g)
+                (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(st, &b->g), "hipStreamEndCapture(batch)");
+        check(hipGraphInstantiate(&b->exec, b->g, nullptr, nullptr, 0),
+              "hipGraphInstantiate(batch)");
+        b->key.assign(items, items + n_items);
+        b->key_state = state;
+        b->key_stream = st;
+        b->valid = true;
+    });
+}
+
+extern "C" void tcoh_batch_destroy(struct tcoh_batch *b)
+{
+    if (!b)
+        return;
+    guarded_void([&] {
+        if (b->device >= 0) {
+            check(hipSetDevice(b->device), "hipSetDevice");
+            for (hipStream_t s : b->streams)
+                (void)hipStreamSynchronize(s);
+        }
+        b->release_graph();
+        for (hipStream_t s : b->streams)
+            (void)hipStreamDestroy(s);
+        for (hipEvent_t e : b->join)
+            (void)hipEventDestroy(e);
+        if (b->fork)
+            (void)hipEventDestroy(b->fork);
+    });
+    delete b;
 }
 
 /* ocl_run's transfers around run_device, with the optional phases. */

@@ -23,6 +23,8 @@ struct KParams {
     float ad_norm, ad_sigma, ad_crit, tlog_crit;
     float *phase;          /* optional: linked phase of every date, planar
                               [N][rows][cols] (radians, NaN where gamma is) */
+    int shp_stripe;        /* XCD stripe widths of the SHP and linking */
+    int link_stripe;       /* launches, in tiles (0: an eighth of a row) */
 };
 
 /* Largest window the SHP mask supports (candidates per pixel). */
@@ -332,37 +334,41 @@ template <int G> __device__ __forceinline__ float2 gbcast2(float2 x, int l)
    consecutive pixels of one row). The hardware deals work-groups to the 8
    XCDs round-robin (id % 8); with a grid of xcd_grid() groups, group id
    maps to logical tile x*per + id/8 of XCD x, and logical tiles run
-   stripe by stripe (stripes nbx/8 tiles wide, top to bottom), so each
-   XCD's L2 holds the windows of one vertical stripe. Returns false for
-   the padding groups. */
+   stripe by stripe (stripes sb tiles wide, top to bottom), so each XCD's
+   L2 holds the windows of a few neighbouring stripes. sb is the launch's
+   tuned stripe width (KParams shp_stripe / link_stripe); 0 takes an eighth
+   of the row, one stripe per XCD; nbx makes every XCD a band of rows.
+   Returns false for the padding groups. */
 #define NXCD 8
 
-__host__ __device__ __forceinline__ int xcd_stripe(int nbx)
+__host__ __device__ __forceinline__ int xcd_stripe(int nbx, int want)
 {
-    return (nbx + NXCD - 1) / NXCD;
+    const int sb = want > 0 ? want : (nbx + NXCD - 1) / NXCD;
+
+    return sb < nbx ? sb : nbx;
 }
 
-__host__ __device__ __forceinline__ int xcd_logical(int R, int nbx)
+__host__ __device__ __forceinline__ int xcd_logical(int R, int nbx, int want)
 {
-    const int sb = xcd_stripe(nbx);
+    const int sb = xcd_stripe(nbx, want);
 
     return ((nbx + sb - 1) / sb) * R * sb;
 }
 
-__host__ __forceinline__ int xcd_grid(int R, int nbx)
+__host__ __forceinline__ int xcd_grid(int R, int nbx, int want)
 {
-    const int per = (xcd_logical(R, nbx) + NXCD - 1) / NXCD;
+    const int per = (xcd_logical(R, nbx, want) + NXCD - 1) / NXCD;
 
     return per * NXCD;
 }
 
-__device__ __forceinline__ bool xcd_tile(int bid, int R, int nbx, int *row,
+__device__ __forceinline__ bool xcd_tile(int bid, int R, int nbx, int want, int *row,
                                          int *tx)
 {
-    const int nl = xcd_logical(R, nbx);
+    const int nl = xcd_logical(R, nbx, want);
     const int per = (nl + NXCD - 1) / NXCD;
     const int l = (bid % NXCD) * per + bid / NXCD;
-    const int sb = xcd_stripe(nbx);
+    const int sb = xcd_stripe(nbx, want);
 
     if (l >= nl)
         return false;

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp
                                                   (EMI ? gi * S::CH_STRIDE : 0));
     int r, tx;
 
-    if (!xcd_tile(blockIdx.x, nrows, (kp.cols + PPW - 1) / PPW, &r, &tx))
+    if (!xcd_tile(blockIdx.x, nrows, (kp.cols + PPW - 1) / PPW, kp.link_stripe, &r, &tx))
         return;
     const int x = tx * PPW + gi;
     const bool live = x < kp.cols;
@@ -334,7 +334,7 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
                          const EmiQueue &q, hipStream_t s)
 {
     typedef LinkShape<N> S;
-    const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW));
+    const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW, kp.link_stripe));
     const int scratch = link_scratch_bytes<N>(kp, precision);
     const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::PPW * S::CH_STRIDE : 0);
 

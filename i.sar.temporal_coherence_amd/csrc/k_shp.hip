@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
     const int n = kp.n;
     int y, tx;
 
-    if (!xcd_tile(blockIdx.x, kp.rows, (kp.cols + WAVE - 1) / WAVE, &y, &tx))
+    if (!xcd_tile(blockIdx.x, kp.rows, (kp.cols + WAVE - 1) / WAVE, kp.shp_stripe, &y, &tx))
         return;
     const int x = tx * WAVE + threadIdx.x;
 
@@ -349,7 +349,7 @@ static void launch_shp_nb(const KParams &kp, const float *ampT,
                           const float *adtab, uint32_t *mask, int *count,
                           hipStream_t s)
 {
-    const dim3 grid(xcd_grid(kp.rows, (kp.cols + WAVE - 1) / WAVE));
+    const dim3 grid(xcd_grid(kp.rows, (kp.cols + WAVE - 1) / WAVE, kp.shp_stripe));
 
     if (kp.test == 0)
         hipLaunchKernelGGL((k_shp<NB, 0>), grid, dim3(WAVE), 0, s, kp, ampT,

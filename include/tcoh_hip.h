@@ -367,6 +367,60 @@ void tcoh_run_device_noalloc(struct ocl *o, const void *slc, const void *amp,
    Extension. */
 void tcoh_set_graph(struct ocl *o, int on);
 
+/* Launch shapes of a handle: tiles[0], tiles[1] the number of XCD stripes
+   a row is cut into by the SHP and linking launches (0 = 8, one stripe per
+   XCD; 1 = the whole row, every XCD a band of rows), tiles[2] the output
+   pixels of one linking launch (0 = the default, about a million). ocl_open takes
+   them from the tuned table compiled into the library (tiles.h, by stack
+   depth, estimator and form); tcoh_set_tiles overrides it (NULL goes back
+   to the table). Only the speed depends on them, not the maps. Extension:
+   the tuner sweeps them through these calls. */
+void tcoh_set_tiles(struct ocl *o, const int tiles[3]);
+void tcoh_get_tiles(const struct ocl *o, int tiles[3]);
+
+/* ---- Batches of stacks ----
+   Several stacks (a handle each: own depth, settings and precision) on
+   one device, run as one unit of work: each item is one strip with
+   tcoh_run_device's buffers. tcoh_batch_run forks from stream (NULL: the
+   first item's handle stream) onto the batch's own streams, enqueues every
+   strip in a longest-first list schedule of tcoh_strip_cost over them, and
+   joins back into stream; it returns without synchronising. Items must be
+   on one device and have distinct handles; the maps equal tcoh_run_device's
+   for each item. In graph mode (tcoh_batch_set_graph) the first call with
+   given items and stream runs eagerly and captures the whole batch into
+   one HIP graph, which later calls with the same items, stream, precisions
+   and launch shapes replay with one hipGraphLaunch. Calls on one batch, and
+   on the handles it runs, must not overlap. Extension: the reference runs
+   one stack per module invocation (main.c:724-825). */
+struct tcoh_batch_item {
+    struct ocl *o;
+    const void *slc, *amp, *valid;
+    int padded_rows, rows, cols;
+    void *coh, *count, *pairs;
+};
+struct tcoh_batch;
+
+/* streams: concurrent lanes of the batch (0: 4, the HIP runtime's hardware
+   queues per process). */
+struct tcoh_batch *tcoh_batch_create(int streams);
+void tcoh_batch_set_graph(struct tcoh_batch *b, int on);
+void tcoh_batch_run(struct tcoh_batch *b, const struct tcoh_batch_item *items,
+                    int n_items, void *stream);
+/* The last run's schedule: order[k] the item issued k-th, lane[i] the
+   stream of item i. */
+void tcoh_batch_plan(const struct tcoh_batch *b, int *order, int *lane);
+void tcoh_batch_destroy(struct tcoh_batch *b);
+
+/* Estimated work of a strip for the schedule: output pixels x (W N^2 +
+   N^3), W the window's candidates (the bound of the SHP count L that the
+   covariance's L N^2 takes) and N^3 the eigen solve. */
+double tcoh_strip_cost(const struct settings *s, int rows, int cols);
+/* Longest processing time first: items by cost, largest first (ties in
+   item order), each to the least loaded of `streams` lanes (ties to the
+   lowest); order[k] the item issued k-th, lane[i] the lane of item i. */
+void tcoh_batch_schedule(int n_items, const double *cost, int streams,
+                         int *order, int *lane);
+
 /* Device time of the last run's stages in milliseconds: [0] preparation
    (planar amplitudes, and the hi/lo split in the mixed forms), [1] SHP
    selection, [2] the fused covariance + linking kernel, [3] reserved (0);

@@ -16,6 +16,8 @@ import numpy as np
 import pytest
 
 import synth
+from test_gpu_parity import M2C_BAND
+from test_oracle import TOLERANCE
 
 pytestmark = pytest.mark.gpu
 
@@ -56,6 +58,38 @@ def prepare_on_device(tcoh, i, q, row0, prow, halo, geometry=None, heights=None)
         out = (slc.to_array(np.float32), amp.to_array(np.float32),
                valid.to_array(np.uint8))
     return out
+
+
+def meets_oracle(cref, fp64, n, opts, i, q, maps, precision, ph=None, en=None):
+    """A region's maps (coh, count, pairs (npairs, px)) against the oracle
+    on the host-prepared whole region (cref.prepare_strip, with the
+    geometric phase ph removed where given): SHP counts equal to the FP32
+    restatement's; gamma and the pair coherences within TOLERANCE of the
+    FP64 oracle with the same NaN pattern. With the bias correction the
+    pairs must be all pairs: their magnitudes give the oracle this run's
+    decisions inside the form's m2c band."""
+    _, rows, cols = i.shape
+    ha, hr = opts["window"][0] // 2, opts["window"][1] // 2
+    prow, pc = rows + 2 * ha, cols + 2 * hr
+    slc, amp, valid = cref.prepare_strip(i, q, -ha, prow, hr, ph, en)
+    so = cref.settings(n, **opts)
+    _, ocount, _, mask, _ = cref.run(so, pc, slc, amp, valid, prow, rows, cols)
+    coh, count, pairs = maps
+    assert np.array_equal(count, ocount), "SHP counts differ from the reference kernel"
+    adopt = None
+    if opts.get("bias"):
+        assert opts.get("pairs") == "all"
+        adopt = pairs.T.astype(np.float64)
+    c64, p64, _, _ = fp64.run(slc, ocount, mask, cref.settings_dict(so), pc, rows, cols,
+                              band=M2C_BAND[precision], adopt=adopt)
+    assert np.array_equal(np.isnan(coh), np.isnan(c64))
+    err = np.nanmax(np.abs(coh - c64))
+    assert err <= TOLERANCE, "max |d gamma| %.3g" % err
+    if pairs is not None and np.isfinite(p64).any():
+        assert np.array_equal(np.isnan(pairs.T), np.isnan(p64))
+        perr = np.nanmax(np.abs(pairs.T - p64))
+        assert perr <= TOLERANCE, "max |d pair| %.3g" % perr
+    return int(np.sum(np.isfinite(coh)))
 
 
 def ulps(a, b):
@@ -164,11 +198,11 @@ def same_maps(got, want):
 
 
 @pytest.mark.parametrize("precision", ["plain", "f16"])
-def test_region_stream_equals_one_strip(tcoh, cref, precision):
+def test_region_stream_equals_one_strip(tcoh, cref, fp64, precision):
     """tcoh_run_region (strips staged, prepared and estimated in a
     pipeline) gives the maps of one ocl_run over the whole host-prepared
     region, bit for bit, for several strip heights (reference
-    test:118-135: strips are seamless)."""
+    test:118-135: strips are seamless), and they meet the oracle."""
     n, rows, cols = 8, 70, 33
     ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
     i, q = planes(tcoh, n, rows, cols, seed=31)
@@ -178,15 +212,18 @@ def test_region_stream_equals_one_strip(tcoh, cref, precision):
     with tcoh.Backend(s, pc, platform=0, device=0, precision=precision) as be:
         want = be.run(slc_h, amp_h, valid_h, prow, rows, cols)
         for strip_rows in (16, 23, 0):
-            same_maps(be.run_region(i, q, strip_rows=strip_rows), want)
-    assert np.sum(want[1] >= s.min_looks) > 100
+            got = be.run_region(i, q, strip_rows=strip_rows)
+            same_maps(got, want)
+    opts = dict(window=WINDOW, min_shp=9, pairs="all", bias=True)
+    assert meets_oracle(cref, fp64, n, opts, i, q, got, precision) > 100
 
 
-def test_region_stream_with_geometric_phase(tcoh, cref):
+def test_region_stream_with_geometric_phase(tcoh, cref, fp64):
     """With the flat-earth / topographic phase removed at elevation heights:
     the streamed region equals the whole region prepared on the device
     (the same strip kernel over the region's full planes) and estimated in
-    one strip."""
+    one strip, and meets the oracle on the region prepared on the host with
+    the oracle's phases (the rotations agree to a float rounding)."""
     n, rows, cols = 6, 58, 40
     ha, hr = WINDOW[0] // 2, WINDOW[1] // 2
     i, q = planes(tcoh, n, rows, cols, seed=41)
@@ -211,16 +248,20 @@ def test_region_stream_with_geometric_phase(tcoh, cref):
         want = (coh.to_array(np.float32), count.to_array(np.int32),
                 pairs.to_array(np.float32, (s.npairs, rows * cols)))
         for strip_rows in (13, 0):
-            same_maps(be.run_region(i, q, geometry=g, heights=heights,
-                                    strip_rows=strip_rows), want)
+            got = be.run_region(i, q, geometry=g, heights=heights, strip_rows=strip_rows)
+            same_maps(got, want)
     assert want[1][10 * cols + 11] == -1
+    ph = cref.pixel_phases(grid, hs, rows, cols, elevation=heights, h0=50.0)
+    en = np.isnan(heights).astype(np.uint8)
+    opts = dict(window=WINDOW, min_shp=9, pairs="consecutive")
+    assert meets_oracle(cref, fp64, n, opts, i, q, got, "f16", ph=ph, en=en) > 100
 
 
-def test_raw_region_writes_the_module_maps(tcoh, tmp_path):
+def test_raw_region_writes_the_module_maps(tcoh, cref, fp64, tmp_path):
     """The module's loop without GRASS: raw I/Q date planes mapped in place,
     streamed through the GPU, written as maps with GRASS's null words, the
     reference's pair-map names and main.c:921-922's summary; the values are
-    those of the in-memory region run."""
+    those of the in-memory region run, which meets the oracle."""
     import datetime as dt
 
     api = tcoh.api
@@ -239,6 +280,8 @@ def test_raw_region_writes_the_module_maps(tcoh, tmp_path):
             tmp_path, "coh", shp_count="shp", pairs_prefix="p", starts=starts,
             strip_rows=17)
     assert names[0] == "p_20230112_20230124" and len(names) == n - 1
+    opts = dict(window=WINDOW, min_shp=9, pairs="consecutive")
+    assert meets_oracle(cref, fp64, n, opts, i, q, (coh, count, pairs), "f16") > 100
     nan = np.isnan(coh)
     f = np.fromfile(tmp_path / "coh.raw", np.uint32)
     assert np.all(f[nan] == 0xFFFFFFFF) and np.array_equal(f[~nan], coh.view(np.uint32)[~nan])

@@ -18,7 +18,7 @@ def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     # "type name(" at a declaration; "double (*pos)[3]" is a field, not one
-    found = re.findall(r"\b(?:void|int|double|float|char|ocl)\s*\**\s*"
+    found = re.findall(r"\b(?:void|int|double|float|char|ocl|tcoh_[a-z_]+)\s*\**\s*"
                        r"([a-z_][a-z0-9_]*)\s*\(", text)
     return sorted(set(found))
 
@@ -79,6 +79,53 @@ def test_c_caller_links_against_the_library(tcoh, tmp_path):
     ref.write_text(C_REFERENCE_TYPES)
     subprocess.run([gcc, "-std=c11", "-Wall", "-Werror", "-fsyntax-only", str(ref),
                     "-I", os.path.join(ROOT, "include")], check=True)
+
+
+# Headers of the shape the module compiles module/geometry.c against: the
+# fields of GRASS's struct Cell_head and of local_proto.h's struct epoch
+# that geometry_init reads (a compile fixture, not the real headers).
+GIS_H = r"""
+struct Cell_head { int rows, cols; double ew_res, ns_res, north, south, east, west; };
+"""
+LOCAL_PROTO_H = r"""
+#include "tcoh_hip.h"
+#define MAX_DATES 64
+struct epoch { char basename[256]; struct Cell_head head; char *meta; };
+void geometry_init(struct geometry *g, const struct epoch *epochs, int n, int ref,
+                   const struct orbit *orbits, const struct Cell_head *region,
+                   double hmin, double hmax, double h0);
+"""
+
+
+def test_module_geometry_adapter_links_against_the_library(tcoh, tmp_path):
+    """module/geometry.c, the drop-in for the module's geometry.c, compiles
+    against headers of the module's shape with geometry_init's reference
+    signature (local_proto.h:96-99) and links against the library, which
+    supplies what it calls and geometry_phases / geometry_free."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    (tmp_path / "grass").mkdir()
+    (tmp_path / "grass" / "gis.h").write_text(GIS_H)
+    (tmp_path / "local_proto.h").write_text(LOCAL_PROTO_H)
+    lib = os.path.dirname(tcoh.api.LIB_PATH)
+    obj = tmp_path / "geometry.o"
+    subprocess.run([gcc, "-std=c11", "-Wall", "-Werror", "-c",
+                    os.path.join(ROOT, "i.sar.temporal_coherence_amd", "module", "geometry.c"),
+                    "-I", str(tmp_path), "-I", os.path.join(ROOT, "include"), "-o", str(obj)],
+                   check=True)
+    so = tmp_path / "libmodule.so"
+    subprocess.run([gcc, "-shared", str(obj), "-L", lib, "-ltcoh_hip", "-Wl,-rpath," + lib,
+                    "-Wl,--no-undefined", "-o", str(so)], check=True)
+    syms = subprocess.run(["nm", "-D", str(so)], check=True, capture_output=True,
+                          text=True).stdout
+    assert re.search(r" T geometry_init$", syms, re.M)
+    for name in ("tcoh_scene_from_metadata", "tcoh_geometry_init"):
+        assert re.search(r" U %s$" % name, syms, re.M), name
+    for name in ("geometry_phases", "geometry_free"):
+        assert hasattr(tcoh.api.lib(), name)
 
 
 def test_settings_layout_matches_reference_struct(tcoh, cref):
