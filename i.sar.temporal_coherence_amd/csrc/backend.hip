@@ -307,6 +307,37 @@ static void tile_lookup(struct ocl *o)
     o->tiles[2] = best && best->band_px > 0 ? best->band_px : TILE_BAND_PX;
 }
 
+/* ocl_open's max_alloc: main.c cuts strips so that the samples (8N
+   bytes a padded pixel) and the pair maps (4P bytes an output pixel) each
+   stay within max_alloc (main.c:731-742). A strip of S pixels needs about
+   S * F bytes on the device in all, F the per-pixel footprint of every
+   buffer (inputs, outputs, SHP masks, date-planar amplitudes and, for the
+   split forms, which a handle may switch to after ocl_open, the 16-bit
+   samples); EMI adds its queues, sized by the linking band. max_alloc =
+   usable * max(8N, 4P) / F therefore keeps such a strip within 90 % of the
+   free memory. The SHP kernel indexes the date planes with 32-bit offsets,
+   which caps a strip at INT32_MAX / bucket pixels. */
+static size_t strip_alloc_limit(const struct settings &s, int band_px, size_t free_bytes)
+{
+    const double n = s.ndates, nb = shp_bucket(s.ndates), nt = n * (n - 1) / 2;
+    const double words = (s.win_az * s.win_rg + 31) / 32;
+    double f = 12 * n + 1 + 4 * nb + 4 + 4 + 4.0 * s.npairs + 4 * words;
+    double fixed = 0;
+
+    if (s.test == SHP_TLOG)
+        f += 8;
+    if (s.emi)
+        fixed = (double)band_px * (8 * nt + 8 * n + 12);
+    else
+        f += 8 * nb + 2;
+    const double usable = 0.9 * (double)free_bytes - fixed;
+    const double big = std::max(8 * n, 4.0 * s.npairs);
+    const double lim = usable > 0 ? usable * big / f : 0.0;
+    const double cap = (double)(INT32_MAX / shp_bucket(s.ndates)) * 8 * n;
+
+    return (size_t)std::min(lim, cap);
+}
+
 static bool eig_supported(int n);
 static hipError_t launch_link(const KParams &kp, const float2 *slc,
                               const uint2 *hl, const short *ex, const uint32_t *mask,
@@ -395,20 +426,8 @@ extern "C" struct ocl *ocl_open(int platform, int device,
                   "hipStreamCreate");
             check(hipGetDeviceProperties(&pr, device), "hipGetDeviceProperties");
             check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
-            /* The strip's samples (8N bytes a pixel) take about 40 % of
-               what one strip needs on the device (amplitudes twice,
-               masks, outputs, T scratch). */
-            /* ... and the SHP kernel indexes the strip's date planes with
-               32-bit offsets: samples of at most INT32_MAX / bucket
-               pixels (main.c sizes strips by max_alloc / (8N) pixels). */
-            if (max_alloc) {
-                const size_t cap = (size_t)(INT32_MAX / shp_bucket(s->ndates)) *
-                                   s->ndates * 8;
-
-                *max_alloc = (size_t)((double)fr * 0.4);
-                if (*max_alloc > cap)
-                    *max_alloc = cap;
-            }
+            if (max_alloc)
+                *max_alloc = strip_alloc_limit(*s, o->tiles[2], fr);
             return o;
         },
         nullptr);

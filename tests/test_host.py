@@ -362,3 +362,25 @@ def test_geometry_phases_reproduce_tensor_polynomials(tcoh, cref):
     want = ((1 - u) * (1 - v) * one[1, 2, 0] + (1 - u) * v * one[1, 3, 0] +
             u * (1 - v) * one[2, 2, 0] + u * v * one[2, 3, 0])
     assert np.allclose(a, want, rtol=1e-14)
+
+
+def test_linking_device_functions_are_inlined():
+    """Every device function of the linking kernels' headers is forced
+    inline. These functions take references into the group's LDS and the
+    lane's private arrays; an out-of-line EMI inversion routine once
+    received them as flat pointers and its accesses faulted the GPU
+    (memory aperture violation in the N=30 EMI linking kernel), so no
+    function there may become a call again."""
+    csrc = os.path.join(ROOT, "i.sar.temporal_coherence_amd", "csrc")
+    bad = []
+    for name in ("common.h", "cov.h", "eig.h", "link.h", "refine.h", "k_link.hip"):
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(csrc, name)).read(), flags=re.S)
+        # a declaration: template header (optional) then qualifiers up to the name's "("
+        for m in re.finditer(r"((?:__host__\s+)?__device__[^;{()]*?)\b([A-Za-z_]\w*)\s*\(",
+                             text):
+            quals = m.group(1)
+            if "__global__" in quals or "__shared__" in quals:
+                continue
+            if "__forceinline__" not in quals:
+                bad.append("%s: %s" % (name, m.group(2)))
+    assert not bad, bad
