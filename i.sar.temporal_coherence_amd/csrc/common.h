@@ -33,6 +33,8 @@ struct KParams {
 /* Lanes of one work-group: every kernel runs one wavefront per group. */
 #define WAVE 64
 
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
 /* Order LDS traffic between the lanes of the (single) wavefront of a
    work-group: the DS instructions of one wavefront execute in issue order,
    so a compiler-level fence is all that is needed, and unlike s_barrier it

@@ -199,7 +199,6 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4_t __attribute__((ext_vector_type(4)));
 
 template <int BF16>

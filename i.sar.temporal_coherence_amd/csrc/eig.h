@@ -58,7 +58,7 @@
    16-byte word a row), which the inverse iteration's arrays replace. */
 template <int N> struct EigLds {
     union {
-        float4 vw[N];
+        float4 vw[(N + 3) & ~3];
         struct {
             float u0[N], u1[N], u2[N], l[N], y[N];
             int piv[N];
@@ -102,47 +102,49 @@ template <int N> __device__ __forceinline__ int lpos(int i, int j)
     return j * N - j * (j - 1) / 2 + (i - j);
 }
 
-/* One row of the Householder update or of its matrix-vector product for
-   G = 16, with the row operand broadcast inside the multiply-adds: VOP2
-   v_fmac_f32 with DPP row_newbcast:L reads lane L of the row as src0.
-   The leading s_nop 1 gives the two wait states a DPP read needs after a
-   VALU write of its source, and one a VALU read needs after a
-   transcendental write; nothing inside the block writes a broadcast
-   source. */
+/* The group's copy of the pixel's N x N Hermitian matrix: lane lg holds
+   the columns c = lg + m*G (m < CPL), both triangles, real and imaginary
+   parts apart, rows in groups of four (one float4 per group), the layout of
+   the accumulator of v_mfma_f32_4x4x1f32: in each block of four adjacent
+   lanes, that instruction's result register i of lane j is (its A operand
+   in the block's lane i) x (its B operand in lane j) + C. One instruction
+   thus adds a rank-1 term to four rows of every lane's column. Rows N ..
+   4*NG-1 are padding. */
+template <int N, int CPL> struct HMat {
+    static constexpr int NG = (N + 3) / 4;
+    float4_t re[CPL][NG], im[CPL][NG];
+
+    __device__ __forceinline__ float2 get(int m, int r) const
+    {
+        return make_float2(re[m][r >> 2][r & 3], im[m][r >> 2][r & 3]);
+    }
+    __device__ __forceinline__ void set(int m, int r, float2 v)
+    {
+        re[m][r >> 2][r & 3] = v.x;
+        im[m][r >> 2][r & 3] = v.y;
+    }
+};
+
+/* c + a_row x b_col on the 4 x 4 blocks (FP32 matrix cores). */
+__device__ __forceinline__ float4_t rank1x4(float a, float b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+/* One row of the Householder matrix-vector product for G = 16, with the
+   row operand broadcast inside the multiply-adds: VOP2 v_fmac_f32 with
+   DPP row_newbcast:L reads lane L of the row as src0. The leading s_nop 1
+   gives the two wait states a DPP read needs after a VALU write of its
+   source, and one a VALU read needs after a transcendental write; nothing
+   inside the block writes a broadcast source. */
 #define TCOH_FMAC_BC(D, S, B, L)                                               \
     "v_fmac_f32_dpp " D ", " S ", " B " row_newbcast:" L                        \
-    " row_mask:0xf bank_mask:0xf\n"
-/* the same with the lane's own operand negated by the DPP src1 modifier */
-#define TCOH_FMAC_BC_NEG(D, S, B, L)                                           \
-    "v_fmac_f32_dpp " D ", " S ", -" B " row_newbcast:" L                       \
     " row_mask:0xf bank_mask:0xf\n"
 #define TCOH_MUL_BC(D, S, B, L)                                                \
     "v_mul_f32_dpp " D ", " S ", " B " row_newbcast:" L                         \
     " row_mask:0xf bank_mask:0xf\n"
 
 template <int L> struct RowOps {
-    /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c) for one column slot, with
-       (wcx, wcy, vcx, vcy) the lane's own w_c, v_c:
-       ax += -(vx*wcx + vy*wcy + wx*vcx + wy*vcy),
-       ay += -vy*wcx + vx*wcy - wy*vcx + wx*vcy */
-    static __device__ __forceinline__ void update1(float &ax, float &ay, float vx,
-                                                   float vy, float wx, float wy,
-                                                   float wcx, float wcy, float vcx,
-                                                   float vcy)
-    {
-        asm volatile("s_nop 1\n"
-                     TCOH_FMAC_BC_NEG("%0", "%2", "%6", "%10")
-                     TCOH_FMAC_BC_NEG("%0", "%3", "%7", "%10")
-                     TCOH_FMAC_BC_NEG("%0", "%4", "%8", "%10")
-                     TCOH_FMAC_BC_NEG("%0", "%5", "%9", "%10")
-                     TCOH_FMAC_BC_NEG("%1", "%3", "%6", "%10")
-                     TCOH_FMAC_BC("%1", "%2", "%7", "%10")
-                     TCOH_FMAC_BC_NEG("%1", "%5", "%8", "%10")
-                     TCOH_FMAC_BC("%1", "%4", "%9", "%10")
-                     : "+v"(ax), "+v"(ay)
-                     : "v"(vx), "v"(vy), "v"(wx), "v"(wy), "v"(wcx), "v"(wcy),
-                       "v"(vcx), "v"(vcy), "n"(L));
-    }
     /* The first row of the partial sums: written, not accumulated. */
     static __device__ __forceinline__ void matvec1_first(float &sr, float &sia,
                                                          float &sib, float vx,
@@ -171,54 +173,22 @@ template <int L> struct RowOps {
     }
 };
 
-/* One row r > k of a 64-lane group's reduction, with (v_r, w_r) read from
-   LDS as t: the matrix-vector partial sums, and the rank-2 update (signs on
-   the lane's own operands, as in the broadcast form). */
-template <int N>
-__device__ __forceinline__ void matvec_row64(int j, int k, float4 t, const float2 (&a)[N],
-                                             float &sr, float &sia, float &sib)
+/* One row r > k of a 64-lane group's matrix-vector product, with v_r read
+   from LDS as t.xy. */
+__device__ __forceinline__ void matvec_row64(bool live, float4 t, float2 a, float &sr,
+                                             float &sia, float &sib)
 {
-    if (j <= k || j >= N)
+    if (!live)
         return;
-    sr = fmaf(t.x, a[j].x, sr);
-    sr = fmaf(t.y, a[j].y, sr);
-    sia = fmaf(t.y, a[j].x, sia);
-    sib = fmaf(t.x, a[j].y, sib);
-}
-
-template <int N>
-__device__ __forceinline__ void update_row64(int r, int k, float4 t, float2 (&a)[N],
-                                             float2 nw, float2 nv, float wcy, float vcy)
-{
-    if (r <= k || r >= N)
-        return;
-    float ax = a[r].x, ay = a[r].y;
-
-    ax = fmaf(t.x, nw.x, ax);
-    ax = fmaf(t.y, nw.y, ax);
-    ax = fmaf(t.z, nv.x, ax);
-    ax = fmaf(t.w, nv.y, ax);
-    ay = fmaf(t.y, nw.x, ay);
-    ay = fmaf(t.x, wcy, ay);
-    ay = fmaf(t.w, nv.x, ay);
-    ay = fmaf(t.z, vcy, ay);
-    a[r] = make_float2(ax, ay);
+    sr = fmaf(t.x, a.x, sr);
+    sr = fmaf(t.y, a.y, sr);
+    sia = fmaf(t.y, a.x, sia);
+    sib = fmaf(t.x, a.y, sib);
 }
 
 #define TCOH_ROW_CASES(F)                                                      \
     F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13)  \
     F(14) F(15)
-
-__device__ __forceinline__ void row_update1(int l, float &ax, float &ay, float2 v,
-                                            float2 w, float2 wc, float2 vc)
-{
-#define TCOH_CASE(L)                                                           \
-    case L:                                                                    \
-        RowOps<L>::update1(ax, ay, v.x, v.y, w.x, w.y, wc.x, wc.y, vc.x, vc.y); \
-        break;
-    switch (l) { TCOH_ROW_CASES(TCOH_CASE) }
-#undef TCOH_CASE
-}
 
 __device__ __forceinline__ void row_matvec1(int l, bool first, float &sr, float &sia,
                                             float &sib, float2 v, float2 a)
@@ -234,9 +204,9 @@ __device__ __forceinline__ void row_matvec1(int l, bool first, float &sr, float 
 #undef TCOH_CASE
 }
 
-/* Householder reduction of the group's matrix (columns in A) to the real
-   tridiagonal (d, e) in LDS; the Householder vector of step k is left in
-   row k of the columns c > k (A[m][k]), tau_k in LDS.
+/* Householder reduction of the group's matrix A to the real tridiagonal
+   (d, e) in LDS; the Householder vector of step k is left in row k of the
+   columns c > k, tau_k in LDS.
 
    Slot m holds columns m*G .. m*G+G-1; a slot whose columns are all
    finished at step k is skipped at compile time. Finished columns inside
@@ -244,14 +214,21 @@ __device__ __forceinline__ void row_matvec1(int l, bool first, float &sr, float 
    are never read again (this layout keeps Householder vectors in rows,
    not columns), so only the reductions are masked.
 
-   Row-wise operands (alpha, v_j, w_r) are the values of the lane that owns
-   column j (slot j / G, lane j % G) and reach the other lanes of the group
-   by lane broadcasts, not LDS: a step has no LDS traffic on its critical
-   path and no load latency in its inner loops. */
+   Per step: the matrix-vector product w = tau A v on the vector ALUs, the
+   row operand v_j of each row reaching the other lanes as a lane broadcast
+   inside the multiply-adds (G = 16), a broadcast value (G = 8) or an LDS
+   read (G = 64); then the rank-2 update A -= v w^H + w v^H on the matrix
+   cores, eight 4 x 4 rank-1 products for every four rows of a slot (real
+   part: v_r.x(-w_c.x) + v_r.y(-w_c.y) + w_r.x(-v_c.x) + w_r.y(-v_c.y);
+   imaginary part: v_r.y(-w_c.x) + v_r.x w_c.y + w_r.y(-v_c.x) + w_r.x v_c.y),
+   with (v_r, w_r) of the block lane's row read from LDS, zero for rows <= k
+   and padding rows, so that a group of four rows is updated whole. */
 template <int N, int G, int CPL, bool EXACT>
-__device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
-                                               EigLds<N> &sh, int lg)
+__device__ __forceinline__ void tridiagonalize(HMat<N, CPL> &A, EigLds<N> &sh, int lg)
 {
+    constexpr int NG = HMat<N, CPL>::NG;
+    const int q4 = lg & 3;
+
 #pragma unroll
     for (int k = 0; k < N - 1; k++) {
         float part = 0.0f;
@@ -262,15 +239,17 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
 
             if (min(m * G + G - 1, N - 1) < k)
                 continue;
+            const float2 a = A.get(m, k);
+
             if (c == k)
-                sh.d[k] = A[m][k].x;
-            const float x2 = fmaf(A[m][k].x, A[m][k].x, A[m][k].y * A[m][k].y);
+                sh.d[k] = a.x;
+            const float x2 = fmaf(a.x, a.x, a.y * a.y);
 
             part += (c >= k + 2 && c < N) ? x2 : 0.0f;
         }
         const float xn = gsum<G>(part);
         /* alpha = A[k+1][k] = conj(A[k][k+1]), row k of column k + 1 */
-        const float2 ak = gbcast2<G>(A[(k + 1) / G][k], (k + 1) % G);
+        const float2 ak = gbcast2<G>(A.get((k + 1) / G, k), (k + 1) % G);
         const float alr = ak.x, ali = -ak.y;
         float beta, tr_ = 0.0f, ti_ = 0.0f, sr = 0.0f, si = 0.0f;
 
@@ -300,14 +279,15 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             if (min(m * G + G - 1, N - 1) < k + 1)
                 continue;
             /* x_c = A[c][k] = conj(A[k][c]), v_c = x_c / (alpha - beta) */
-            const float xr = A[m][k].x, xi = -A[m][k].y;
+            const float2 a = A.get(m, k);
+            const float xr = a.x, xi = -a.y;
             const float2 xs = make_float2(fmaf(xr, sr, -xi * si),
                                           fmaf(xr, si, xi * sr));
             const bool live = c >= k + 1 && c < N;
 
             vc[m] = c == k + 1 ? make_float2(1.f, 0.f) : (live ? xs : vc[m]);
             if (c >= k + 2)
-                A[m][k] = xs;
+                A.set(m, k, xs);
             act[m] = live ? 1.0f : 0.0f;
         }
         if (lg == 0) {
@@ -357,10 +337,14 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                 const float4 n2 = (j + 6 > k && j + 6 < N) ? sh.vw[j + 6] : c2;
                 const float4 n3 = (j + 7 > k && j + 7 < N) ? sh.vw[j + 7] : c3;
 
-                matvec_row64<N>(j, k, c0, A[0], sr2[0], sia[0], sib[0]);
-                matvec_row64<N>(j + 1, k, c1, A[0], sr2[0], sia[0], sib[0]);
-                matvec_row64<N>(j + 2, k, c2, A[0], sr2[0], sia[0], sib[0]);
-                matvec_row64<N>(j + 3, k, c3, A[0], sr2[0], sia[0], sib[0]);
+                matvec_row64(j > k && j < N, c0, A.get(0, j < N ? j : 0), sr2[0],
+                             sia[0], sib[0]);
+                matvec_row64(j + 1 > k && j + 1 < N, c1, A.get(0, j + 1 < N ? j + 1 : 0),
+                             sr2[0], sia[0], sib[0]);
+                matvec_row64(j + 2 > k && j + 2 < N, c2, A.get(0, j + 2 < N ? j + 2 : 0),
+                             sr2[0], sia[0], sib[0]);
+                matvec_row64(j + 3 > k && j + 3 < N, c3, A.get(0, j + 3 < N ? j + 3 : 0),
+                             sr2[0], sia[0], sib[0]);
                 TCOH_STEP_FENCE();
                 c0 = n0;
                 c1 = n1;
@@ -378,7 +362,7 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
                     if (min(m * G + G - 1, N - 1) < k + 1)
                         continue;
                     row_matvec1(j % G, j == k + 1, sr2[m], sia[m], sib[m], vc[j / G],
-                                A[m][j]);
+                                A.get(m, j));
                 }
                 continue;
             }
@@ -388,10 +372,12 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             for (int m = 0; m < CPL; m++) {
                 if (min(m * G + G - 1, N - 1) < k + 1)
                     continue;
-                sr2[m] = fmaf(v.x, A[m][j].x, sr2[m]);
-                sr2[m] = fmaf(v.y, A[m][j].y, sr2[m]);
-                sia[m] = fmaf(v.y, A[m][j].x, sia[m]);
-                sib[m] = fmaf(v.x, A[m][j].y, sib[m]);
+                const float2 a = A.get(m, j);
+
+                sr2[m] = fmaf(v.x, a.x, sr2[m]);
+                sr2[m] = fmaf(v.y, a.y, sr2[m]);
+                sia[m] = fmaf(v.y, a.x, sia[m]);
+                sib[m] = fmaf(v.x, a.y, sib[m]);
             }
         }
         float2 wc[CPL];
@@ -424,88 +410,51 @@ __device__ __forceinline__ void tridiagonalize(float2 (&A)[CPL][N],
             }
         }
 
-        /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r > k, with the signs
-           on the lane's own operands. */
-        float2 nw[CPL], nv[CPL];
-
+        /* Row operands (v_r, w_r) for the update: every row of the groups
+           the update touches, zero for finished columns (rows <= k) and
+           padding (rows >= N). */
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
-            nw[m] = make_float2(-wc[m].x, -wc[m].y);
-            nv[m] = make_float2(-vc[m].x, -vc[m].y);
+            const int c = lg + m * G;
+            const bool lv = c >= k + 1 && c < N;
+
+            if (m * G < 4 * NG && c < 4 * NG)
+                sh.vw[c] = lv ? make_float4(vc[m].x, vc[m].y, wc[m].x, wc[m].y)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        if constexpr (G == 64) {
-            const int c = lg;
-
-            if (c >= k + 1 && c < N) {
-                sh.vw[c].z = wc[0].x;
-                sh.vw[c].w = wc[0].y;
-            }
-            wave_sync();
-            float4 c0, c1, c2, c3;
-
-            c0 = (0 > k && 0 < N) ? sh.vw[0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            c1 = (1 > k && 1 < N) ? sh.vw[1] : make_float4(0.f, 0.f, 0.f, 0.f);
-            c2 = (2 > k && 2 < N) ? sh.vw[2] : make_float4(0.f, 0.f, 0.f, 0.f);
-            c3 = (3 > k && 3 < N) ? sh.vw[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+        wave_sync();
+        /* A[r][c] -= v_r conj(w_c) + w_r conj(v_c), r > k */
 #pragma unroll
-            for (int r = 0; r < N; r += 4) {
-                const float4 n0 = (r + 4 > k && r + 4 < N) ? sh.vw[r + 4] : c0;
-                const float4 n1 = (r + 5 > k && r + 5 < N) ? sh.vw[r + 5] : c1;
-                const float4 n2 = (r + 6 > k && r + 6 < N) ? sh.vw[r + 6] : c2;
-                const float4 n3 = (r + 7 > k && r + 7 < N) ? sh.vw[r + 7] : c3;
-
-                update_row64<N>(r, k, c0, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
-                update_row64<N>(r + 1, k, c1, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
-                update_row64<N>(r + 2, k, c2, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
-                update_row64<N>(r + 3, k, c3, A[0], nw[0], nv[0], wc[0].y, vc[0].y);
-                TCOH_STEP_FENCE();
-                c0 = n0;
-                c1 = n1;
-                c2 = n2;
-                c3 = n3;
-            }
-            /* vw is read again only after the next step's writes */
-            wave_sync();
-        }
-#pragma unroll
-        for (int r = 0; r < N; r++) {
-            if (G == 64 || r <= k)
+        for (int g0 = 0; g0 < NG; g0++) {
+            if (4 * g0 + 3 <= k)
                 continue;
-            if (G == 16) {
-#pragma unroll
-                for (int m = 0; m < CPL; m++) {
-                    if (min(m * G + G - 1, N - 1) < k + 1)
-                        continue;
-                    row_update1(r % G, A[m][r].x, A[m][r].y, vc[r / G], wc[r / G],
-                                wc[m], vc[m]);
-                }
-                continue;
-            }
-            const float2 v = gbcast2<G>(vc[r / G], r % G);
-            const float2 w = gbcast2<G>(wc[r / G], r % G);
+            const float4 t = sh.vw[4 * g0 + q4];
 
 #pragma unroll
             for (int m = 0; m < CPL; m++) {
                 if (min(m * G + G - 1, N - 1) < k + 1)
                     continue;
-                float ax = A[m][r].x, ay = A[m][r].y;
+                float4_t ar = A.re[m][g0], ai = A.im[m][g0];
 
-                ax = fmaf(v.x, nw[m].x, ax);
-                ax = fmaf(v.y, nw[m].y, ax);
-                ax = fmaf(w.x, nv[m].x, ax);
-                ax = fmaf(w.y, nv[m].y, ax);
-                ay = fmaf(v.y, nw[m].x, ay);
-                ay = fmaf(v.x, wc[m].y, ay);
-                ay = fmaf(w.y, nv[m].x, ay);
-                ay = fmaf(w.x, vc[m].y, ay);
-                A[m][r] = make_float2(ax, ay);
+                ar = rank1x4(t.x, -wc[m].x, ar);
+                ai = rank1x4(t.y, -wc[m].x, ai);
+                ar = rank1x4(t.y, -wc[m].y, ar);
+                ai = rank1x4(t.x, wc[m].y, ai);
+                ar = rank1x4(t.z, -vc[m].x, ar);
+                ai = rank1x4(t.w, -vc[m].x, ai);
+                ar = rank1x4(t.w, -vc[m].y, ar);
+                ai = rank1x4(t.z, vc[m].y, ai);
+                A.re[m][g0] = ar;
+                A.im[m][g0] = ai;
             }
         }
+        /* vw is written again only after every lane has read it */
+        wave_sync();
     }
 #pragma unroll
     for (int m = 0; m < CPL; m++)
         if (lg + m * G == N - 1)
-            sh.d[N - 1] = A[m][N - 1].x;
+            sh.d[N - 1] = A.get(m, N - 1).x;
     wave_sync();
 }
 
@@ -669,7 +618,7 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
 
 /* z = H_0 ... H_{N-2} y for the owned components (tcoh.cl:426-443). */
 template <int N, int G, int CPL>
-__device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
+__device__ __forceinline__ void back_transform(const HMat<N, CPL> &A,
                                                const EigLds<N> &sh, int lg,
                                                float2 (&z)[CPL])
 {
@@ -697,7 +646,7 @@ __device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
             if (c == k + 1)
                 v[m] = make_float2(1.f, 0.f);
             else if (c > k + 1 && c < N)
-                v[m] = A[m][k];
+                v[m] = A.get(m, k);
             sr += v[m].x * z[m].x + v[m].y * z[m].y;
             si += v[m].x * z[m].y - v[m].y * z[m].x;
         }
@@ -721,7 +670,7 @@ __device__ __forceinline__ void back_transform(const float2 (&A)[CPL][N],
 /* Eigenvector (owned components in z) of rank idx of the group's
    Hermitian matrix A (destroyed). */
 template <int N, int G, int CPL, bool EXACT>
-__device__ __forceinline__ void hermitian_eigenvector(float2 (&A)[CPL][N],
+__device__ __forceinline__ void hermitian_eigenvector(HMat<N, CPL> &A,
                                                       EigLds<N> &sh, int idx,
                                                       int lg, float2 (&z)[CPL],
                                                       float *lam_out = nullptr,

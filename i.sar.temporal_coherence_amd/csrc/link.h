@@ -27,7 +27,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
 {
     deferred = false;
     constexpr int CPL = (N + G - 1) / G;
-    float2 A[CPL][N];
+    HMat<N, CPL> A;
 
     /* EVD keeps T in registers from here on; EMI reads it again after the
        inversion, so that it is not held live through it. */
@@ -37,10 +37,12 @@ __device__ __forceinline__ float link_group(const KParams &kp,
             const int c = lg + m * G;
 
 #pragma unroll
-            for (int r = 0; r < N; r++) {
-                A[m][r] = make_float2(r == c ? 1.f : 0.f, 0.f);
-                if (est && c < N)
-                    A[m][r] = load_t<N>(tp, r, c);
+            for (int r = 0; r < 4 * HMat<N, CPL>::NG; r++) {
+                float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
+
+                if (est && c < N && r < N)
+                    t = load_t<N>(tp, r, c);
+                A.set(m, r, t);
             }
         }
     }
@@ -61,7 +63,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
             for (int r = 0; r < N; r++) {
                 const float2 t = EMI ? (est && c < N ? load_t<N>(tp, r, c)
                                                      : make_float2(0.f, 0.f))
-                                     : A[m][r];
+                                     : A.get(m, r);
 
                 if (c < N && r != c && (t.x != 0.0f || t.y != 0.0f))
                     adj[m] |= 1ull << r;
@@ -116,12 +118,15 @@ __device__ __forceinline__ float link_group(const KParams &kp,
             const int c = lg + m * G;
 
 #pragma unroll
-            for (int r = 0; r < N; r++) {
+            for (int r = 0; r < 4 * HMat<N, CPL>::NG; r++) {
                 float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
 
-                if (est && c < N)
-                    t = load_t<N>(tp, r, c);
-                A[m][r] = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
+                if (r < N) {
+                    if (est && c < N)
+                        t = load_t<N>(tp, r, c);
+                    t = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
+                }
+                A.set(m, r, t);
             }
         }
         float lam, norm;
