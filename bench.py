@@ -44,8 +44,8 @@ METRIC_GEO = ("Mpixel/s at N=60, 21x7 KS+EVD, geometric phase removed: mixed-pre
               "plain FP32 GPU, one GPU's 1024 x 25000 share")
 # HBM bytes per k_link launch (512 rows x 2048 px, split FP16, N=30) from
 # the two counter passes of profiles/k_link_n30_2048x2048_kernel_stats.md:
-# (269,745 KiB x 2.0 + 635,639 KiB x 1.0) x 1024 B.
-TRAFFIC_F16_N30 = (269745 * 2.0 + 635639 * 1.0) * 1024
+# (236,923 KiB x 2.0 + 615,287 KiB x 1.0) x 1024 B.
+TRAFFIC_F16_N30 = (236923 * 2.0 + 615287 * 1.0) * 1024
 PEAK_FP64 = 78.6e12    # MI355X FP64 vector (TFLOP/s)
 PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
 PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
