@@ -534,6 +534,10 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         }
         wave_sync();
         float rsr[NBK][4], rsc[NBK];
+        /* FP16: the diagonal sums squares of FP16 values (at least 2^-48
+           when not zero), never an FP32 denormal, so the hardware rsq
+           needs no range scaling; BF16 samples are unscaled FP32 values */
+        auto rsq = [](float v) { return BF16 ? rsqrtf(v) : __builtin_amdgcn_rsqf(v); };
 
 #pragma unroll
         for (int a = 0; a < NBK; a++) {
@@ -541,11 +545,11 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
             for (int r = 0; r < 4; r++) {
                 const float dv = diag[a * 16 + 4 * g + r];
 
-                rsr[a][r] = dv > 0.0f ? rsqrtf(dv) : 0.0f;
+                rsr[a][r] = dv > 0.0f ? rsq(dv) : 0.0f;
             }
             const float dv = diag[a * 16 + i16];
 
-            rsc[a] = dv > 0.0f ? rsqrtf(dv) : 0.0f;
+            rsc[a] = dv > 0.0f ? rsq(dv) : 0.0f;
         }
         const bool bias = kp.bias && looks > 1;
         const float fl = (float)looks;
@@ -563,11 +567,12 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                     float re = cre[t][r] * sc, im = cim[t][r] * sc;
 
                     if (bias) {
-                        /* tcoh.cl:620-633 */
+                        /* tcoh.cl:620-633; where m2c > 0, m2 > 1/L is far
+                           from the denormals (hardware rsq) */
                         const float m2 = re * re + im * im;
                         const float m2c = (fl * m2 - 1.0f) * inv_l1;
                         const float f = (m2 > 0.0f && m2c > 0.0f)
-                                            ? sqrt_fast(m2c) * rsqrtf(m2)
+                                            ? sqrt_fast(m2c) * __builtin_amdgcn_rsqf(m2)
                                             : 0.0f;
 
                         re *= f;

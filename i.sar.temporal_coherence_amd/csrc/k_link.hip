@@ -79,9 +79,13 @@ __device__ __forceinline__ void store_phases(const KParams &kp, const EigLds<N> 
    launch: one code object serves the three forms, so the linking code is
    compiled once per depth and estimator); EMI ignores it (plain, FP64
    sums). EMI pixels whose |T| fails the Cholesky pivot test go to queue 1
-   (T, NT float2 each, and the output pixel), in place of a result. */
+   (T, NT float2 each, and the output pixel), in place of a result.
+   EVD kernels run two wavefronts per SIMD (256 registers each) at every
+   depth: above 32 dates that spills (1.4 KB a lane at N=60) but the second
+   wavefront hides more than the spills cost (N=45: 49.7 -> 32.8 ms, N=60:
+   33.9 -> 29.8 ms on 768^2 and 512^2 frames); EMI keeps one (512). */
 template <int N, int EMI>
-__global__ __launch_bounds__(64, (EMI || N > 32) ? 1 : 2) void k_link(KParams kp,
+__global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
                                                const uint2 *__restrict__ hl,
                                                const short *__restrict__ ex,

@@ -138,6 +138,22 @@ def test_nulls_and_zero_samples(tcoh, cref, fp64, precision):
     assert np.isnan(coh[20, 20]) and count[20, 20] == -1
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_high_dynamic_range(tcoh, cref, fp64, precision):
+    """Point scatterers 60 dB above the distributed background (amplitude
+    x1000 at every date) next to a block 60 dB below it: every form holds
+    the 1e-4 bar, the FP16 split through its per-pixel scale and per-window
+    rescale (a bright pixel costs the windows without it no precision)."""
+    stack = synth.ds_stack(12, 0.6, rows=40, cols=48, seed=8)
+    stack[:, 12, 15] *= 1000.0
+    stack[:, 25, 30] *= 1000.0
+    stack[:, 30:33, 5:9] *= 1e-3
+    stack[:, 8:11, 40] *= 1e-3
+    check_against_oracles(tcoh, cref, fp64, stack,
+                          dict(window=(9, 5), min_shp=5, bias=True, pairs="all"),
+                          precision, tol_for(precision))
+
+
 @pytest.mark.parametrize("precision", ["plain", "f16"])
 def test_strips_are_seamless(tcoh, precision):
     """reference test:118-135: a tiny memory budget gives several strips and
