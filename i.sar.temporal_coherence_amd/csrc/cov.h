@@ -297,7 +297,11 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     constexpr int NBK = NB / 16;             /* 16-date blocks */
     constexpr int NUP = NBK * (NBK + 1) / 2; /* upper block pairs */
     constexpr int NT = N * (N - 1) / 2;
-    constexpr int DEPTH = NBK <= 2 ? 2 : 1;  /* chunks in flight */
+    /* chunks in flight: at 64 date slots the buffers compete with the
+       accumulators for registers, and from 48 dates on a single buffer
+       (no prefetch: fewer spills at two wavefronts per SIMD) is faster
+       (N=52, 60, 64: 4, 7, 2 % on 512^2-768^2 frames; N=45: 3 % slower) */
+    constexpr int DEPTH = NBK <= 2 ? 2 : (N >= 48 ? 0 : 1);
     const int lane = threadIdx.x;
     const int kw = kp.wa * kp.wr, nw = kp.words;
     const int lstride = mfma_list_stride(kw);
@@ -613,6 +617,12 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                 break;
             fetch(min(c + 4, last), b1);
             step(c + 2, b2);
+        }
+    }
+    else if (DEPTH == 0) {
+        for (int c = 0; c < total; c++) {
+            fetch(c, b0);
+            step(c, b0);
         }
     }
     else {
