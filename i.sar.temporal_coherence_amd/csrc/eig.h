@@ -524,7 +524,12 @@ __device__ __forceinline__ float tridiag_eigenvalue(const EigLds<N> &sh,
 }
 
 /* Inverse iteration of tcoh.cl:340-410; every lane of the group computes
-   the same vector, y ends in LDS. */
+   the same vector, y ends in LDS. Above 32 dates the factorisation and the
+   substitutions stop every four steps (TCOH_STEP_FENCE) so that the
+   compiler does not hoist every step's LDS operands ahead of the serial
+   chain: with them all in registers the N=60 kernel spilled 1.2 KB a lane
+   inside these loops (340 bytes with the fences; N=60 512^2: 28.2 ->
+   19.5 ms). */
 template <int N, bool EXACT>
 __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
                                                     float lambda, float norm,
@@ -566,6 +571,8 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
         sh.u0[k] = EXACT ? a0 : rcpf(a0);
         sh.u1[k] = a1;
         sh.u2[k] = a2;
+        if (N > 32 && k % 4 == 3)
+            TCOH_STEP_FENCE();
     }
     if (fabsf(dd) < tiny)
         dd = dd < 0.0f ? -tiny : tiny;
@@ -588,6 +595,8 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
                 y[k + 1] = t;
             }
             y[k + 1] -= sh.l[k] * y[k];
+            if (N > 32 && k % 4 == 3)
+                TCOH_STEP_FENCE();
         }
 #pragma unroll
         for (int k = N - 1; k >= 0; k--) {
@@ -598,6 +607,8 @@ __device__ __forceinline__ void tridiag_eigenvector(EigLds<N> &sh,
             if (k + 2 < N)
                 v -= sh.u2[k] * y[k + 2];
             y[k] = EXACT ? v / sh.u0[k] : v * sh.u0[k];
+            if (N > 32 && k % 4 == 0)
+                TCOH_STEP_FENCE();
         }
 #pragma unroll
         for (int k = 0; k < N; k++)
