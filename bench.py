@@ -495,9 +495,11 @@ def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, n, s_opts, y0, x0, s
     sd = cref.settings_dict(so)
     ocoh, ocount, _, mask, _ = cref.run(so, pc_c, slc3, amp3, val3, pr_c, size, size,
                                         nthreads=min(16, os.cpu_count() or 1))
+    log("accuracy: oracle SHP counts and FP32 gamma of the %d^2 crop" % size)
     s32 = np.ascontiguousarray(slc3, np.float32).reshape(-1, n, 2)
     slc_c = s32[..., 0].astype(np.float64) + 1j * s32[..., 1].astype(np.float64)
     c64_plain = fp64.run(slc_c, ocount, mask, sd, pc_c, size, size)[0]
+    log("accuracy: FP64 oracle of the crop")
     d32 = np.abs(ocoh - c64_plain)
     out = {"oracle_fp32": {"max_abs_dgamma": float(np.nanmax(d32)),
                            "pixels_over_1e-4": int(np.sum(d32 > 1e-4))}}
@@ -513,6 +515,7 @@ def crop_accuracy(tcoh, cref, fp64, slc, amp, valid, pcols, n, s_opts, y0, x0, s
         c64, p64, near, _ = fp64.run(slc_c, ocount, mask, sd, pc_c, size, size, band=band,
                                      adopt=adopt if sd["bias"] else None, z_out=z64,
                                      zmag_out=zmag)
+        log("accuracy: FP64 oracle with the %s form's bias decisions" % form)
         d = np.abs(cc - c64)
         dp = np.abs(pc - p64)
         both = np.isfinite(phc.T) & np.isfinite(z64.real)
@@ -834,6 +837,8 @@ def main():
                             "at most OMP_NUM_THREADS when that is set",
             "sample": "%d x %d px band of the same frame (%.1f s), FP32 C restatement "
                       "of the reference kernel, OpenMP over rows" % (band, cols, dt)}
+        log("cpu baseline: %.4f Mpx/s on %d threads (%.1f s)"
+            % (out["cpu_baseline"]["value"], threads, dt))
     print(json.dumps(out))
 
 
