@@ -42,13 +42,28 @@ C4_STACKS = [(8, 2048, 3072, "ks"), (12, 2048, 2048, "tlog"), (20, 1536, 2048, "
              (30, 1024, 1024, "tlog"), (45, 768, 768, "ks"), (60, 512, 512, "tlog")]
 METRIC_GEO = ("Mpixel/s at N=60, 21x7 KS+EVD, geometric phase removed: mixed-prec vs "
               "plain FP32 GPU, one GPU's 1024 x 25000 share")
-# HBM bytes per k_link launch (512 rows x 2048 px, split FP16, N=30) from
-# the two counter passes of profiles/k_link_n30_2048x2048_kernel_stats.md:
-# (236,923 KiB x 2.0 + 615,287 KiB x 1.0) x 1024 B.
-TRAFFIC_F16_N30 = (236923 * 2.0 + 615287 * 1.0) * 1024
+# HBM bytes per k_link launch (512 rows x 2048 px, N=30) of the split form
+# TRAFFIC_FORM, from the two counter passes of
+# profiles/k_link_n30_2048x2048_kernel_stats.md (FETCH_SIZE KiB x 2.0 +
+# WRITE_SIZE KiB x 1.0) x 1024 B; None until measured for the current build.
+TRAFFIC_FORM = None
+TRAFFIC_N30 = None
 PEAK_FP64 = 78.6e12    # MI355X FP64 vector (TFLOP/s)
 PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
 PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
+
+
+def mfma_per_chunk_of(n, form):
+    """v_mfma_f32_16x16x32 instructions the mixed covariance executes per
+    32-candidate chunk (16*16*32*2 flop each), each product as hi.hi +
+    hi.lo + lo.hi: Re Re^T + Im Im^T on the upper 16 x 16 date block pairs;
+    C_im as X - X^T from X = Im Re^T on all block pairs up to 32 date
+    slots (BF16; FP16 where the pixel's T area holds the transposition
+    tiles, N >= 29), else Im Re^T - Re Im^T on the upper pairs."""
+    nbk = (16 if n <= 16 else (32 if n <= 32 else 64)) // 16
+    nup = nbk * (nbk + 1) // 2
+    xt = nbk <= 2 and (form == "bf16" or (nbk == 2 and n >= 29))
+    return 6 * nup + (3 * nbk * nbk if xt else 6 * nup)
 
 
 def log(*a):
@@ -551,7 +566,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="f16", choices=["f16", "bf16"])
+    ap.add_argument("--precision", default="bf16", choices=["f16", "bf16"],
+                    help="the timed mixed form (config 2 names split BF16)")
+    ap.add_argument("--no-alt", action="store_true",
+                    help="skip the other split form (reported beside the headline)")
     ap.add_argument("--rows", type=int, default=None,
                     help="default 2048 (configs 2, 3), 1024 (config 5)")
     ap.add_argument("--cols", type=int, default=None,
@@ -651,16 +669,18 @@ def main():
     torch.cuda.synchronize()
     log("scene %dx%d N=%d ready in %.1f s" % (rows, cols, n, time.perf_counter() - t0))
 
+    emi = s_opts["estimator"] == "emi"
     forms = {}
     maps = {}
     band_of = {}
     band_same = {}
     graph = {}
-    order = [args.precision] + ([] if args.no_plain else ["plain"])
+    alt = "f16" if args.precision == "bf16" else "bf16"
+    order = [args.precision] + ([] if args.no_alt or emi else [alt]) + \
+            ([] if args.no_plain else ["plain"])
     crop = min(args.crop, rows, cols)
     cy0 = max(0, rows // 2 - crop // 2)
     cx0 = max(0, cols // 2 - crop // 2)
-    emi = s_opts["estimator"] == "emi"
     for form in order:
         steps = args.steps if form == args.precision else max(2, args.steps // 2)
         ms, st_ms, nl = run_form(tcoh, s, pcols, bufs, rows, cols, form, steps,
@@ -719,14 +739,9 @@ def main():
     achieved = flop / max(1.0, nl[dom]) / avg_launch_s
     peak = flop / t_bound
     # the HBM traffic was measured (rocprofv3 counters) on this workload only
-    traffic_measured = (args.config == 2 and args.precision == "f16"
+    traffic_measured = (args.config == 2 and args.precision == TRAFFIC_FORM
                         and rows == 2048 and cols == 2048)
-    # Matrix-core work the mixed form executes: per estimated pixel,
-    # ceil(L/32) chunks of 32 candidates, each 12 v_mfma_f32_16x16x32 per
-    # upper 16 x 16 date block pair (hi.hi, hi.lo, lo.hi of Re Re^T, Im Im^T,
-    # Im Re^T, Re Im^T), 16*16*32*2 flop each.
-    nbk = (16 if n <= 16 else (32 if n <= 32 else 64)) // 16
-    mfma_per_chunk = 12 * nbk * (nbk + 1) // 2
+    mfma_per_chunk = mfma_per_chunk_of(n, args.precision)
     mfma = None
     if not emi:
         executed = float(np.sum(np.ceil(looks / 32.0))) * mfma_per_chunk * 16 * 16 * 32 * 2
@@ -740,7 +755,7 @@ def main():
                         "the same kernel is in profiles/"}
     roofline = {"bound": bound, "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": TRAFFIC_F16_N30 if traffic_measured else None,
+                "traffic": TRAFFIC_N30 if traffic_measured else None,
                 "traffic_source": ("profiles/k_link_n30_2048x2048_kernel_stats.md "
                                    "(rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
                                    "calibrated on a 1 GiB copy)") if traffic_measured else None,
@@ -774,12 +789,19 @@ def main():
     if graph:
         out["graph"] = dict(graph, path="tcoh_set_graph: the strip replayed as one "
                                         "captured HIP graph per step")
+    if alt in forms:
+        ams, ast, _, asteps = forms[alt]
+        out[alt] = {"value": npix / (ams * 1e-3) / 1e6, "ms_per_step": ams,
+                    "steps": asteps, "speedup_vs_plain": None,
+                    "stage_ms": dict(zip(stage_names, np.round(ast, 3).tolist()))}
     if "plain" in forms:
         pms, pst, _, psteps = forms["plain"]
         out["plain"] = {"value": npix / (pms * 1e-3) / 1e6, "ms_per_step": pms,
                         "steps": psteps,
                         "stage_ms": dict(zip(stage_names, np.round(pst, 3).tolist()))}
         out["speedup_vs_plain"] = pms / ms
+        if alt in out:
+            out[alt]["speedup_vs_plain"] = pms / out[alt]["ms_per_step"]
 
     if host_planes is not None:
         # The whole frame from host date planes through the streaming entry

@@ -22,8 +22,10 @@
  *   the window's common one, exactly, by the candidate's exponent below
  *   the window's largest; BF16: O(2^-18 x), unscaled) and every product
  *   is accumulated in FP32 from hi.hi +
- *   hi.lo + lo.hi with v_mfma_f32_16x16x32_{f16,bf16}, for the upper
- *   16 x 16 date blocks only. The accepted candidates are compacted into
+ *   hi.lo + lo.hi with v_mfma_f32_16x16x32_{f16,bf16}: C_re for the upper
+ *   16 x 16 date blocks only; C_im up to 32 date slots as X - X^T from
+ *   X = Im Re^T over all blocks (three products a block pair instead of
+ *   six), above that directly. The accepted candidates are compacted into
  *   LDS lists padded to a multiple of 32 with a zero pixel; A and B
  *   operands share one lane layout, so the candidate order inside a
  *   32-candidate chunk does not matter. Dates are padded to NB with zeros.
@@ -266,13 +268,26 @@ __host__ __device__ __forceinline__ int mfma_list_stride(int kw)
     return (kw + 31) & ~31;
 }
 
-/* LDS bytes of the mixed form for ppw pixels: lists and their factors,
+/* Bytes of the second list area: the FP16 form's factors, or (BF16, up to
+   32 date slots) the transposition tiles of the epilogue, one 16 x 16
+   float tile per upper block pair, whichever is larger. */
+__host__ __device__ __forceinline__ int mfma_area2_bytes(int ppw, int kw, int nb)
+{
+    const int fac = ppw * mfma_list_stride(kw) * 4;
+    const int nbk = nb / 16;
+    const int xt = nb <= 32 ? nbk * (nbk + 1) / 2 * 1024 : 0;
+
+    return fac > xt ? fac : xt;
+}
+
+/* LDS bytes of the mixed form for ppw pixels: lists, the second area,
    the diagonal of C, the pixels' mask words, and a per-pixel table
    (chunks, looks). */
 __host__ __device__ __forceinline__ int mfma_scratch_bytes(int ppw, int kw,
                                                            int words, int nb)
 {
-    return 2 * ppw * mfma_list_stride(kw) * 4 + nb * 4 + ppw * words * 4 + ppw * 8;
+    return ppw * mfma_list_stride(kw) * 4 + mfma_area2_bytes(ppw, kw, nb) + nb * 4 +
+           ppw * words * 4 + ppw * 8;
 }
 
 /* Mixed form over the group's pixels x0 .. x0+PPW-1 of output row y, the
@@ -305,9 +320,19 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     const int lane = threadIdx.x;
     const int kw = kp.wa * kp.wr, nw = kp.words;
     const int lstride = mfma_list_stride(kw);
+    /* Up to 32 date slots: C_im = X - X^T from X = Im Re^T over all
+       block pairs (a, b), which takes 3 split products per pair instead
+       of 6 per upper pair (NBK = 2: 12 MFMAs a chunk instead of 18) and
+       no sign flips; the epilogue transposes X(b, a) through LDS: BF16 in
+       the second list area (it has no factors), FP16 in the pixel's own T
+       area before T is written, where that holds the tiles (N >= 29). */
+    constexpr int XT_BYTES = NBK * (NBK + 1) / 2 * 1024;
+    constexpr bool XT = NBK <= 2 && (BF16 || NT * 8 - 8 >= XT_BYTES);
     int *list = reinterpret_cast<int *>(scratch);
     uint32_t *fac = reinterpret_cast<uint32_t *>(scratch + PPW * lstride * 4);
-    float *diag = reinterpret_cast<float *>(scratch + 2 * PPW * lstride * 4);
+    float *xtl = reinterpret_cast<float *>(fac);
+    float *diag = reinterpret_cast<float *>(scratch + PPW * lstride * 4 +
+                                            mfma_area2_bytes(PPW, kw, NB));
     uint32_t *mw = reinterpret_cast<uint32_t *>(diag + NB);
     /* chunks and looks of pixel q at ptab[q], ptab[PPW + q]: read back with
        a uniform address when q is only known at run time (a register
@@ -366,8 +391,13 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                 const int cand = origin + dy * kp.w + (b - dy * kp.wr);
 
                 lq[base + rank] = cand;
-                if (!BF16)
-                    emax = max(emax, (int)ex[cand]);
+                if (!BF16) {
+                    /* the exponent beside it, for the factor below */
+                    const int e = ex[cand];
+
+                    fac[q * lstride + base + rank] = (uint32_t)e;
+                    emax = max(emax, e);
+                }
             }
             base += __popcll(m64);
         }
@@ -376,13 +406,13 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         if (!BF16) {
             /* FP16: each candidate's factor 2^-(its exponent below the
                window's largest accepted one), by which step() rescales its
-               hi and lo parts */
+               hi and lo parts, in place of the exponent */
             uint32_t *fq = fac + q * lstride;
 
             emax = gmaxi<WAVE>(emax);
             wave_sync();
             for (int t = lane; t < lpad; t += WAVE)
-                fq[t] = t < base ? f16_pow2_neg_pair(min(emax - (int)ex[lq[t]], LIST_SHIFT_MAX))
+                fq[t] = t < base ? f16_pow2_neg_pair(min(emax - (int)fq[t], LIST_SHIFT_MAX))
                                  : F16_ONE_PAIR;
         }
         for (int t = base + lane; t < lpad; t += WAVE)
@@ -402,11 +432,17 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         return;
 
     const int i16 = lane & 15, g = lane >> 4;
-    float4_t cre[NUP], cim[NUP];
+    /* xim[a * NBK + b] = X(a, b) (XT only; cim is then formed in the
+       epilogue) */
+    constexpr int NX = XT ? NBK * NBK : 1;
+    float4_t cre[NUP], cim[NUP], xim[NX];
 
 #pragma unroll
     for (int t = 0; t < NUP; t++)
         cre[t] = cim[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < NX; t++)
+        xim[t] = float4_t{0.f, 0.f, 0.f, 0.f};
 
     /* Pixel and chunk of flat chunk c (pixels without estimate have no
        chunks). */
@@ -487,8 +523,10 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 #pragma unroll
         for (int bk = 0; bk < NBK; bk++) {
             op[bk] = make_opnd(sv[bk]);
-            nrh[bk] = op[bk].rh ^ 0x80008000u;
-            nrl[bk] = op[bk].rl ^ 0x80008000u;
+            if (!XT) {
+                nrh[bk] = op[bk].rh ^ 0x80008000u;
+                nrl[bk] = op[bk].rl ^ 0x80008000u;
+            }
         }
         /* hi.hi + hi.lo + lo.hi of each product, the accumulators of a
            step independent of each other. */
@@ -498,18 +536,35 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         _Pragma("unroll") for (int a = 0; a < NBK; a++)                        \
             _Pragma("unroll") for (int b = a; b < NBK; b++, t++) { STMT; }     \
     }
-        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rh, cre[t]);
-                   cim[t] = mfma<BF16>(op[a].ih, op[b].rh, cim[t]))
-        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rl, cre[t]);
-                   cim[t] = mfma<BF16>(op[a].ih, op[b].rl, cim[t]))
-        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rl, op[b].rh, cre[t]);
-                   cim[t] = mfma<BF16>(op[a].il, op[b].rh, cim[t]))
-        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].ih, cre[t]);
-                   cim[t] = mfma<BF16>(nrh[a], op[b].ih, cim[t]))
-        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].il, cre[t]);
-                   cim[t] = mfma<BF16>(nrh[a], op[b].il, cim[t]))
-        TCOH_UPPER(cre[t] = mfma<BF16>(op[a].il, op[b].ih, cre[t]);
-                   cim[t] = mfma<BF16>(nrl[a], op[b].ih, cim[t]))
+#define TCOH_ALL(STMT)                                                         \
+    _Pragma("unroll") for (int a = 0; a < NBK; a++)                            \
+        _Pragma("unroll") for (int b = 0; b < NBK; b++) { STMT; }
+        if constexpr (XT) {
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rh, cre[t]))
+            TCOH_ALL(xim[a * NBK + b] = mfma<BF16>(op[a].ih, op[b].rh, xim[a * NBK + b]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].ih, cre[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rl, cre[t]))
+            TCOH_ALL(xim[a * NBK + b] = mfma<BF16>(op[a].ih, op[b].rl, xim[a * NBK + b]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].il, cre[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rl, op[b].rh, cre[t]))
+            TCOH_ALL(xim[a * NBK + b] = mfma<BF16>(op[a].il, op[b].rh, xim[a * NBK + b]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].il, op[b].ih, cre[t]))
+        }
+        else {
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rh, cre[t]);
+                       cim[t] = mfma<BF16>(op[a].ih, op[b].rh, cim[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rh, op[b].rl, cre[t]);
+                       cim[t] = mfma<BF16>(op[a].ih, op[b].rl, cim[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].rl, op[b].rh, cre[t]);
+                       cim[t] = mfma<BF16>(op[a].il, op[b].rh, cim[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].ih, cre[t]);
+                       cim[t] = mfma<BF16>(nrh[a], op[b].ih, cim[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].ih, op[b].il, cre[t]);
+                       cim[t] = mfma<BF16>(nrh[a], op[b].il, cim[t]))
+            TCOH_UPPER(cre[t] = mfma<BF16>(op[a].il, op[b].ih, cre[t]);
+                       cim[t] = mfma<BF16>(nrl[a], op[b].ih, cim[t]))
+        }
+#undef TCOH_ALL
 #undef TCOH_UPPER
 
         if (--left > 0)
@@ -536,7 +591,43 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                         diag[a * 16 + i16] = v;
                     }
         }
+        /* the transposition area (16-byte aligned) */
+        float *xa = xtl;
+
+        if (!BF16) {
+            const uintptr_t u = reinterpret_cast<uintptr_t>(tq);
+
+            xa = reinterpret_cast<float *>((u + 15) & ~static_cast<uintptr_t>(15));
+        }
+        if constexpr (XT) {
+            /* X(b, a) of each upper pair (a, b) into tile t of the area,
+               column-major (element (i, j) at j * 16 + i) */
+            int t = 0;
+
+#pragma unroll
+            for (int a = 0; a < NBK; a++)
+#pragma unroll
+                for (int b = a; b < NBK; b++, t++)
+                    *reinterpret_cast<float4_t *>(xa + t * 256 + i16 * 16 + 4 * g) =
+                        xim[b * NBK + a];
+        }
         wave_sync();
+        if constexpr (XT) {
+            /* C_im(a, b)[i][j] = X(a, b)[i][j] - X(b, a)[j][i], with
+               i = 4g + r and j = i16 */
+            int t = 0;
+
+#pragma unroll
+            for (int a = 0; a < NBK; a++)
+#pragma unroll
+                for (int b = a; b < NBK; b++, t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        cim[t][r] = xim[a * NBK + b][r] - xa[t * 256 + (4 * g + r) * 16 + i16];
+            /* FP16: the area is the T being written below */
+            if (!BF16)
+                wave_sync();
+        }
         float rsr[NBK][4], rsc[NBK];
         /* FP16: the diagonal sums squares of FP16 values (at least 2^-48
            when not zero), never an FP32 denormal, so the hardware rsq
@@ -588,6 +679,9 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 #pragma unroll
         for (int u = 0; u < NUP; u++)
             cre[u] = cim[u] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < NX; u++)
+            xim[u] = float4_t{0.f, 0.f, 0.f, 0.f};
         if (c + 1 < total) {
             int off;
 
