@@ -273,12 +273,12 @@ static int env_precision()
     const char *e = getenv("TCOH_PRECISION");
 
     if (!e || !*e)
-        return TCOH_MIXED_F16;
+        return TCOH_MIXED_BF16;
     if (!strcmp(e, "plain"))
         return TCOH_PLAIN;
-    if (!strcmp(e, "bf16"))
-        return TCOH_MIXED_BF16;
-    return TCOH_MIXED_F16;
+    if (!strcmp(e, "f16"))
+        return TCOH_MIXED_F16;
+    return TCOH_MIXED_BF16;
 }
 
 /* The tuned shapes of a handle's estimator, form and depth (tiles.h):

@@ -269,7 +269,8 @@ enum tcoh_precision {
 };
 
 /* Select the covariance arithmetic of a handle (default: the environment
-   variable TCOH_PRECISION = plain | f16 | bf16, else f16). */
+   variable TCOH_PRECISION = plain | f16 | bf16, else bf16, the split form
+   with no scaling of the samples). */
 void tcoh_set_precision(struct ocl *o, int precision);
 
 /* main.c:52-127 (read_strip) without the GRASS row reads: build the padded

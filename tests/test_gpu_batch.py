@@ -3,7 +3,7 @@
 Six stacks of depth N in {8, 12, 20, 30, 45, 60} (the batched workload's
 depths; KS for three, tlog for the other three, EVD, bias correction,
 21x7 windows, per-stack seeds and grid shapes), run as one batch in the
-split FP16 form. Every stack's maps must equal its own tcoh_run_device
+split BF16 form (the library's default). Every stack's maps must equal its own tcoh_run_device
 maps bit for bit, and those are held to the oracle as the single-stack
 parity tests hold them (SHP counts bit-exact against the FP32 restatement
 of the reference kernel, gamma and pair coherences within 1e-4 of the
@@ -64,7 +64,7 @@ def same(m1, m2):
 @pytest.fixture(scope="module")
 def batch_setup(tcoh):
     stacks = [device_stack(tcoh, c) for c in STACKS]
-    bes = [tcoh.Backend(s, pcols, platform=0, device=0, precision="f16")
+    bes = [tcoh.Backend(s, pcols, platform=0, device=0, precision="bf16")
            for _, s, _, pcols, _ in stacks]
     yield stacks, bes
     for be in bes:
@@ -90,7 +90,7 @@ def test_batch_equals_each_stack_and_the_oracle(tcoh, cref, fp64, batch_setup):
         be.run_device(*item(be, c, prow, b)[1:])
         assert same(maps(b, s.npairs, npx), m), "N=%d: batch differs from its own run" % c["n"]
         # ocl_run's maps are those of tcoh_run_device; they meet the oracle
-        coh, count, pairs = check_against_oracles(tcoh, cref, fp64, stack, c["opts"], "f16")
+        coh, count, pairs = check_against_oracles(tcoh, cref, fp64, stack, c["opts"], "bf16")
         assert np.array_equal(count, m[1])
         assert np.array_equal(coh.view(np.uint32), m[0].view(np.uint32))
         assert np.array_equal(pairs.reshape(-1).view(np.uint32),

@@ -154,7 +154,7 @@ def test_high_dynamic_range(tcoh, cref, fp64, precision):
                           precision, tol_for(precision))
 
 
-@pytest.mark.parametrize("precision", ["plain", "f16"])
+@pytest.mark.parametrize("precision", PRECISIONS)
 def test_strips_are_seamless(tcoh, precision):
     """reference test:118-135: a tiny memory budget gives several strips and
     the same maps as one strip."""
@@ -197,7 +197,7 @@ def test_depth30_scene_crop(tcoh, cref, fp64, precision):
                           tol_for(precision))
 
 
-@pytest.mark.parametrize("precision", ["plain", "f16"])
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("estimator", ["evd", "emi"])
 @pytest.mark.parametrize("n", [33, 45, 64])
 def test_deep_stacks(tcoh, cref, fp64, n, estimator, precision):

@@ -197,7 +197,7 @@ def same_maps(got, want):
         assert np.array_equal(np.asarray(g).view(np.uint32), np.asarray(w).view(np.uint32))
 
 
-@pytest.mark.parametrize("precision", ["plain", "f16"])
+@pytest.mark.parametrize("precision", ["plain", "f16", "bf16"])
 def test_region_stream_equals_one_strip(tcoh, cref, fp64, precision):
     """tcoh_run_region (strips staged, prepared and estimated in a
     pipeline) gives the maps of one ocl_run over the whole host-prepared
@@ -254,7 +254,8 @@ def test_region_stream_with_geometric_phase(tcoh, cref, fp64):
     ph = cref.pixel_phases(grid, hs, rows, cols, elevation=heights, h0=50.0)
     en = np.isnan(heights).astype(np.uint8)
     opts = dict(window=WINDOW, min_shp=9, pairs="consecutive")
-    assert meets_oracle(cref, fp64, n, opts, i, q, got, "f16", ph=ph, en=en) > 100
+    # (the handle's default form: split BF16)
+    assert meets_oracle(cref, fp64, n, opts, i, q, got, "bf16", ph=ph, en=en) > 100
 
 
 def test_raw_region_writes_the_module_maps(tcoh, cref, fp64, tmp_path):
@@ -281,7 +282,7 @@ def test_raw_region_writes_the_module_maps(tcoh, cref, fp64, tmp_path):
             strip_rows=17)
     assert names[0] == "p_20230112_20230124" and len(names) == n - 1
     opts = dict(window=WINDOW, min_shp=9, pairs="consecutive")
-    assert meets_oracle(cref, fp64, n, opts, i, q, (coh, count, pairs), "f16") > 100
+    assert meets_oracle(cref, fp64, n, opts, i, q, (coh, count, pairs), "bf16") > 100
     nan = np.isnan(coh)
     f = np.fromfile(tmp_path / "coh.raw", np.uint32)
     assert np.all(f[nan] == 0xFFFFFFFF) and np.array_equal(f[~nan], coh.view(np.uint32)[~nan])
@@ -295,7 +296,7 @@ def test_raw_region_writes_the_module_maps(tcoh, cref, fp64, tmp_path):
     assert mean == pytest.approx(float(np.mean(coh[~nan], dtype=np.float64)), rel=1e-12)
 
 
-@pytest.mark.parametrize("precision", ["plain", "f16"])
+@pytest.mark.parametrize("precision", ["plain", "f16", "bf16"])
 def test_depth60_with_geometric_phase(tcoh, cref, fp64, precision):
     """The full-geometry workload's path on a crop: N=60 dates carrying
     exp(-i phi_geo) of a 0-500 m DEM (phi_geo from the library's FP64 grid

@@ -2,7 +2,8 @@
 tcoh_run_device on the same device buffers, through the C ABI: the same
 maps bit for bit, on two calls in a row, the second with a smaller strip
 (the fixed scratch array keeps the first call's contents), for the split
-FP16 form with bias correction and pairs, and for EMI (queues in the
+FP16 form with bias correction and pairs, the split BF16 form at 30 dates
+with a 21x7 window, and for EMI (queues in the
 scratch); and a strip beyond the scratch is refused with an error."""
 
 import ctypes as C
@@ -25,6 +26,8 @@ def noalloc(tcoh):
 
 CASES = {
     "f16_evd_bias": dict(n=12, precision="f16", window=(9, 5), min_shp=12, bias=True,
+                         pairs="consecutive", estimator="evd", shp_test="ks"),
+    "bf16_evd_n30": dict(n=30, precision="bf16", window=(21, 7), min_shp=30, bias=True,
                          pairs="consecutive", estimator="evd", shp_test="ks"),
     "plain_emi_ad": dict(n=10, precision="plain", window=(9, 5), min_shp=10, bias=True,
                          pairs="all", estimator="emi", shp_test="ad"),
