@@ -1,9 +1,10 @@
 /* Statistically homogeneous pixel selection on gfx950.
  *
  * One lane per output pixel. The strip's sorted amplitudes are first
- * transposed to date-planar order (ampT[k][pixel]) so that the lanes of a
- * wavefront, which hold consecutive pixels, read consecutive addresses for
- * every candidate. Each lane emits the accepted candidates of its window
+ * transposed to planes of four dates (ampT[k/4][pixel][k%4], a float4 a
+ * pixel and plane) so that the lanes of a wavefront, which hold
+ * consecutive pixels, read one contiguous 1 KB block per four dates of
+ * every candidate: one 16-byte load and one address step per four dates. Each lane emits the accepted candidates of its window
  * as a bit mask (dy-major, bit dy*WR+dx, as the reference's candidate
  * order tcoh.cl:572-576) and the SHP count (tcoh.cl:603).
  *
@@ -38,7 +39,8 @@ __device__ __forceinline__ float logf_rounded(float x)
     return (float)log((double)x);
 }
 
-/* ampT[k][pix] = amp[pix][k]; lstat[pix] = (mean, var) of log amplitudes
+/* ampT[k/4][pix][k%4] = amp[pix][k] (dates n..NB-1 zero); lstat[pix] =
+   (mean, var) of log amplitudes
    of valid pixels when the t-test is used. A block stages its PREP_PX
    pixels' amplitudes (contiguous in amp) in LDS with coalesced reads, then
    writes the date planes; a row of the tile is padded to n + 1 words so
@@ -71,8 +73,12 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
     const int pix = p0 + threadIdx.x;
     const float *a = tile + threadIdx.x * n1;
 
-    for (int k = 0; k < shp_bucket(n); k++)
-        ampT[(size_t)k * kp.npad + pix] = k < n ? a[k] : 0.0f;
+    float4 *quad = reinterpret_cast<float4 *>(ampT);
+
+    for (int k = 0; k < shp_bucket(n); k += 4)
+        quad[(size_t)(k >> 2) * kp.npad + pix] =
+            make_float4(k < n ? a[k] : 0.0f, k + 1 < n ? a[k + 1] : 0.0f,
+                        k + 2 < n ? a[k + 2] : 0.0f, k + 3 < n ? a[k + 3] : 0.0f);
     if (kp.test == 2) {
         float mean = 0.0f, var = 0.0f;
 
@@ -95,9 +101,37 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
    loaded once per pixel): accept iff b[j] < ahi[j] for j < N-K and
    alo[j] <= b[j] for j >= K. All N candidate samples are loaded
    unconditionally (clamped index) so the loads issue back to back. */
-/* ampT holds NB planes (n..NB-1 zero), so plane j of a candidate is
-   ampT[cand + j * npad] for every j < NB; npv is npad in a vector
-   register, so the 32-bit plane offsets are per-lane adds. */
+/* The NB dates of pixel pix (n..NB-1 zero) from the planes of four:
+   quad q at (q * npad + pix) float4s; npv is npad in a vector register,
+   so the plane offsets are per-lane adds, one per four dates. */
+template <int NB>
+__device__ __forceinline__ void load_dates(float (&b)[NB], const float *__restrict__ ampT,
+                                           int pix, int npv)
+{
+    const float4 *quad = reinterpret_cast<const float4 *>(ampT);
+    int idx = pix;
+
+#pragma unroll
+    for (int q = 0; q < NB / 4; q++) {
+        const float4 v = quad[idx];
+
+        b[4 * q] = v.x;
+        b[4 * q + 1] = v.y;
+        b[4 * q + 2] = v.z;
+        b[4 * q + 3] = v.w;
+        idx += npv;
+    }
+}
+
+/* Date d of pixel pix (one float of its plane of four). */
+__device__ __forceinline__ float load_date(const float *__restrict__ ampT, int pix, int d,
+                                           int npv)
+{
+    return ampT[((size_t)(d >> 2) * npv + pix) * 4 + (d & 3)];
+}
+
+/* ampT holds NB dates a pixel (n..NB-1 zero), so every load below is in
+   bounds. */
 template <int NB>
 __device__ __forceinline__ bool ks_ok(const float (&ahi)[NB],
                                       const float (&alo)[NB], int n, int kmax,
@@ -106,19 +140,19 @@ __device__ __forceinline__ bool ks_ok(const float (&ahi)[NB],
 {
     float b[NB];
     bool ok = kmax >= 1;
-    int idx = cand;
 
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        b[j] = ampT[idx];
-        idx += npv;
-    }
+    load_dates<NB>(b, ampT, cand, npv);
     /* Dates without a condition carry NaN thresholds: !(b >= NaN) and
        !(NaN > b) hold, and for numbers they are b < ahi and alo <= b
-       (amplitudes of valid pixels are never NaN). */
+       (amplitudes of valid pixels are never NaN). The tests of a plane of
+       four are folded in before the next plane's (a scheduling barrier):
+       hoisted together, their 64 lane masks outgrew the scalar registers. */
 #pragma unroll
-    for (int j = 0; j < NB; j++)
+    for (int j = 0; j < NB; j++) {
         ok &= !(b[j] >= ahi[j]) & !(alo[j] > b[j]);
+        if (j % 4 == 3)
+            __builtin_amdgcn_sched_barrier(0);
+    }
     return ok;
 }
 
@@ -162,9 +196,7 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
     }
     const int npv = in_vgpr(kp.npad);
 
-#pragma unroll
-    for (int i = 0; i < NB; i++)
-        a[i] = ampT[centre + i * npv];
+    load_dates<NB>(a, ampT, centre, npv);
 
     if (TEST == 0) {
         /* ahi[j] = a[j+K] for j < N-K, alo[j] = a[j-K] for K <= j < N,
@@ -174,8 +206,8 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
 
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-            const float h = ampT[centre + min(j + k, n - 1) * npv];
-            const float l = ampT[centre + min(max(j - k, 0), n - 1) * npv];
+            const float h = load_date(ampT, centre, min(j + k, n - 1), npv);
+            const float l = load_date(ampT, centre, min(max(j - k, 0), n - 1), npv);
 
             ahi[j] = j + k < n ? h : nan;
             alo[j] = (j >= k && j < n) ? l : nan;
@@ -210,13 +242,8 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                     int ia = 0, ib = 0, m = 0;
                     float s = 0.0f;
                     float b[NB];
-                    int idx = cand;
 
-#pragma unroll
-                    for (int i = 0; i < NB; i++) {
-                        b[i] = ampT[idx];
-                        idx += npv;
-                    }
+                    load_dates<NB>(b, ampT, cand, npv);
 #pragma unroll
                     for (int i = 0; i < NB; i++)
                         if (i < n)
