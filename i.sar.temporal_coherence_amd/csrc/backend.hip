@@ -595,6 +595,16 @@ static ScratchPlan scratch_plan(const KParams &kp, int precision, int band_px)
     pl.mixed = precision != TCOH_PLAIN && !kp.emi;
     pl.bf16 = precision == TCOH_MIXED_BF16;
     pl.chunk = std::max(1, band_px / kp.cols);
+    if (pl.mixed) {
+        /* a launch's window rows of split samples within LIST_SPAN_MAX */
+        const size_t row_bytes = (size_t)kp.w * nb * 8;
+        const long fit = (long)(LIST_SPAN_MAX / row_bytes) - (kp.wa - 1);
+
+        if (fit < 1)
+            fail("Strip rows of %d padded pixels at %d dates are too wide for the split "
+                 "forms with a %d-line window", kp.w, kp.n, kp.wa);
+        pl.chunk = (int)std::min<long>(pl.chunk, fit);
+    }
     pl.cap = std::min(pl.chunk, kp.rows) * kp.cols;
     pl.bytes[SC_AMPT] = npad * nb * sizeof(float);
     pl.bytes[SC_MASK] = nout * kp.words * sizeof(uint32_t);

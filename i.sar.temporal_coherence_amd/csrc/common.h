@@ -30,6 +30,11 @@ struct KParams {
 /* Largest window the SHP mask supports (candidates per pixel). */
 #define TCOH_MAX_WINDOW 1024
 
+/* Largest span (bytes) of the split-sample rows one linking launch
+   gathers from, its band's window rows: the gathers address them with
+   32-bit offsets below the candidate lists' pad value (0xFFFF0000). */
+#define LIST_SPAN_MAX 0xFFF00000u
+
 /* Lanes of one work-group: every kernel runs one wavefront per group. */
 #define WAVE 64
 

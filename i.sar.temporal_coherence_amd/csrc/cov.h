@@ -236,12 +236,19 @@ __device__ __forceinline__ Opnd make_opnd(const uint2 (&v)[8])
     return o;
 }
 
-/* Candidate lists of the mixed form: the strip pixel index of every
-   accepted candidate, padded to a multiple of 32 with the zero pixel
-   (index npad); in the FP16 form a parallel list holds each candidate's
+/* Candidate lists of the mixed form: the byte offset of every accepted
+   candidate's sample row from the band's first window row (the base of a
+   range-checked buffer resource over the band's rows), padded to a
+   multiple of 32 with LIST_PAD, beyond every band: its loads return zero
+   without touching memory; in the FP16 form a parallel list holds each
+   candidate's
    rescaling factor 2^-(its exponent below the window's largest), as a pair
    of FP16 values. */
 #define LIST_SHIFT_MAX 31
+#define LIST_PAD 0xFFFF0000u
+/* Third descriptor word of a raw buffer resource on gfx9 parts (32-bit
+   data format, no swizzle); loads past the range return zero. */
+#define RAW_BUFFER_WORD3 0x00020000
 #define F16_ONE_PAIR 0x3C003C00u
 
 /* FP16 bits of 2^-s, 0 <= s <= 31 (subnormal from s = 15, zero above
@@ -304,7 +311,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                                               const short *__restrict__ ex,
                                               const uint32_t *__restrict__ mask,
                                               const int *__restrict__ count,
-                                              int y, int x0,
+                                              int y, int x0, int row0, int nrows,
                                               unsigned char *scratch,
                                               float2 *tl,
                                               float *__restrict__ pairs)
@@ -365,6 +372,8 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         }
         int *lq = list + q * lstride;
         const int origin = y * kp.w + x0 + q;
+        /* the band's first window row: list offsets count from it */
+        const int band_px = row0 * kp.w;
         int base = 0, emax = SPLIT_EX_ZERO;
 
         for (int it = 0; it * 64 < kw; it++) {
@@ -390,7 +399,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 
                 const int cand = origin + dy * kp.w + (b - dy * kp.wr);
 
-                lq[base + rank] = cand;
+                lq[base + rank] = (int)((uint32_t)(cand - band_px) * (uint32_t)(NB * 8));
                 if (!BF16) {
                     /* the exponent beside it, for the factor below */
                     const int e = ex[cand];
@@ -416,7 +425,7 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                                  : F16_ONE_PAIR;
         }
         for (int t = base + lane; t < lpad; t += WAVE)
-            lq[t] = kp.npad;
+            lq[t] = (int)LIST_PAD;
         nch[q] = lpad >> 5;
         total += nch[q];
     }
@@ -459,6 +468,14 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
     /* Samples of flat chunk c for this lane: dates bk*16 + i16 of the 8
        candidates of its group g. hl rows hold NB date slots (zero past N),
        so every load is unconditional. */
+    /* The band's sample rows (window rows row0 .. row0 + nrows + wa - 2)
+       as a range-checked buffer: a list offset is one 32-bit add from the
+       lane's date, and the pads (LIST_PAD) load zeros. The host keeps the
+       span within LIST_SPAN_MAX. */
+    const __amdgpu_buffer_rsrc_t rows_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint2 *>(hl + (size_t)row0 * kp.w * NB), 0,
+        (int)((uint32_t)(nrows + kp.wa - 1) * (uint32_t)kp.w * (uint32_t)(NB * 8)),
+        RAW_BUFFER_WORD3);
     auto fetch = [&](int c, uint2(&v)[NBK][8]) {
         int q, off;
 
@@ -470,11 +487,14 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
 
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const uint2 *row = hl + (size_t)(unsigned)idx[e] * NB + i16;
+            const int o = (int)((uint32_t)idx[e] + (uint32_t)(i16 * 8));
 
 #pragma unroll
-            for (int bk = 0; bk < NBK; bk++)
-                v[bk][e] = row[bk * 16];
+            for (int bk = 0; bk < NBK; bk++) {
+                const auto w = __builtin_amdgcn_raw_buffer_load_b64(rows_rsrc, o + bk * 128, 0, 0);
+
+                v[bk][e] = make_uint2(w[0], w[1]);
+            }
         }
     };
 

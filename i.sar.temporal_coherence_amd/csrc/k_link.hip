@@ -124,10 +124,10 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
                                      reinterpret_cast<float *>(scratch) + gi * N,
                                      tl + gi * NT, pairs);
     else if (cov == 1)
-        cov_mfma_band<N, NB, PPW, 0>(kp, hl, ex, mask, count, row0 + r, tx * PPW,
+        cov_mfma_band<N, NB, PPW, 0>(kp, hl, ex, mask, count, row0 + r, tx * PPW, row0, nrows,
                                      scratch, tl, pairs);
     else
-        cov_mfma_band<N, NB, PPW, 1>(kp, hl, ex, mask, count, row0 + r, tx * PPW,
+        cov_mfma_band<N, NB, PPW, 1>(kp, hl, ex, mask, count, row0 + r, tx * PPW, row0, nrows,
                                      scratch, tl, pairs);
     wave_sync();
     EigLds<N> *sh = reinterpret_cast<EigLds<N> *>(scratch);
