@@ -668,7 +668,9 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
         }
         const bool bias = kp.bias && looks > 1;
         const float fl = (float)looks;
+        /* f^2 = m2c / m2 = (L m2 - 1) / ((L - 1) m2) = L/(L-1) - 1/((L-1) m2) */
         const float inv_l1 = bias ? 1.0f / (fl - 1.0f) : 0.0f;
+        const float fa = fl * inv_l1;
         int t = 0;
 
 #pragma unroll
@@ -682,13 +684,13 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                     float re = cre[t][r] * sc, im = cim[t][r] * sc;
 
                     if (bias) {
-                        /* tcoh.cl:620-633; where m2c > 0, m2 > 1/L is far
-                           from the denormals (hardware rsq) */
+                        /* tcoh.cl:620-633: the magnitude scaled by
+                           sqrt(m2c / m2), zero where m2c <= 0; the same
+                           threshold L m2 = 1 decided on f^2 (m2 = 0: the
+                           hardware reciprocal is +inf, f^2 -inf, f 0) */
                         const float m2 = re * re + im * im;
-                        const float m2c = (fl * m2 - 1.0f) * inv_l1;
-                        const float f = (m2 > 0.0f && m2c > 0.0f)
-                                            ? sqrt_fast(m2c) * __builtin_amdgcn_rsqf(m2)
-                                            : 0.0f;
+                        const float f2 = fmaf(-inv_l1, __builtin_amdgcn_rcpf(m2), fa);
+                        const float f = f2 > 0.0f ? sqrt_fast(f2) : 0.0f;
 
                         re *= f;
                         im *= f;
@@ -697,8 +699,11 @@ __device__ __forceinline__ void cov_mfma_band(const KParams &kp,
                         tq[upper_index(N, i, j)] = make_float2(re, im);
                 }
 #pragma unroll
-        for (int u = 0; u < NUP; u++)
-            cre[u] = cim[u] = float4_t{0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < NUP; u++) {
+            cre[u] = float4_t{0.f, 0.f, 0.f, 0.f};
+            if (!XT) /* (XT: cim is formed anew in each epilogue) */
+                cim[u] = float4_t{0.f, 0.f, 0.f, 0.f};
+        }
 #pragma unroll
         for (int u = 0; u < NX; u++)
             xim[u] = float4_t{0.f, 0.f, 0.f, 0.f};
