@@ -324,7 +324,10 @@ void tcoh_run_region(struct ocl *o, int nrows, int ncols, const float *i,
 
 /* ocl_run on device-resident buffers (hipMalloc'd, same layouts), enqueued
    on stream (a hipStream_t; NULL is the handle's own stream). Returns
-   without synchronising. */
+   without synchronising: no host wait anywhere in the strip, EMI included.
+   Every call on a handle uses that handle's scratch buffers, so calls on
+   one handle must run in order (one stream, or streams the caller orders
+   with events); handles are independent. */
 void tcoh_run_device(struct ocl *o, const void *slc, const void *amp,
                      const void *valid, int padded_rows, int rows, int cols,
                      void *coh, void *count, void *pairs, void *stream);
