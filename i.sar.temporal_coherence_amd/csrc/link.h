@@ -12,6 +12,51 @@
    of the first queue kernel, the Jacobi pseudo-inverse of the second. */
 enum { INV_CHOLESKY = 0, INV_GAUSS_JORDAN = 1, INV_JACOBI = 2 };
 
+/* Column c of the pixel's T (lane c of a group of G lanes) from the packed
+   strict upper triangle: T_rc at upper_index(N, r, c) = K(r) + c above
+   the diagonal, the conjugate of upper_index(N, c, r) = Q(c) + r below
+   it, both a lane base plus a compile-time offset once r is. Rows entirely
+   above or below the columns of slot m take one read; rows among them read
+   both and select. Reads whose values are discarded may leave the pixel's
+   T but not the group's LDS: every index is >= 0 (Q(c) >= -1 for c < N,
+   and r >= 1 where it is read) and at most G past the pixel's last entry,
+   inside the scratch area that follows the group's T areas. */
+template <int N, int G> struct TCol {
+    const float2 *tp, *colp, *rowp;
+    int c;
+
+    __device__ __forceinline__ TCol(const float2 *t, int col)
+        : tp(t), colp(t + col),
+          /* (padding columns, c >= N, read from the start: their base
+             would fall below the pixel's T for small N) */
+          rowp(t + (col < N ? col * N - col * (col + 1) / 2 - col - 1 : 0)), c(col)
+    {
+    }
+    /* T_rc, r < N, r != c for a column c < N (the diagonal gives 1) */
+    __device__ __forceinline__ float2 get(int m, int r) const
+    {
+        const float2 id = make_float2(r == c ? 1.f : 0.f, 0.f);
+        const int kr = r * N - r * (r + 1) / 2 - r - 1;
+
+        if (r < m * G)
+            return colp[kr]; /* every column of the slot is right of r */
+        if (r > m * G + G - 1) {
+            const float2 v = rowp[r]; /* every column left of r */
+
+            return make_float2(v.x, -v.y);
+        }
+        if (r == 0) {
+            /* only the first slot's lanes: c > 0 above the diagonal */
+            const float2 v = tp[c > 0 ? c - 1 : 0];
+
+            return c > 0 ? v : id;
+        }
+        const float2 up = colp[kr], lo = rowp[r];
+
+        return r < c ? up : (r > c ? make_float2(lo.x, -lo.y) : id);
+    }
+};
+
 /* gamma of the group's pixel, NaN when it is not estimated or (with the
    bias correction) its dates are not linked. EMI: a pixel whose |T| the
    INV step cannot invert (Cholesky pivot test, condition bound) is not
@@ -34,15 +79,14 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     if (!EMI) {
 #pragma unroll
         for (int m = 0; m < CPL; m++) {
-            const int c = lg + m * G;
+            const TCol<N, G> col(tp, lg + m * G);
+            const bool use = est && lg + m * G < N;
 
 #pragma unroll
             for (int r = 0; r < 4 * HMat<N, CPL>::NG; r++) {
-                float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
+                const float2 id = make_float2(r == lg + m * G ? 1.f : 0.f, 0.f);
 
-                if (est && c < N && r < N)
-                    t = load_t<N>(tp, r, c);
-                A.set(m, r, t);
+                A.set(m, r, use && r < N ? col.get(m, r) : id);
             }
         }
     }
@@ -122,7 +166,14 @@ __device__ __forceinline__ float link_group(const KParams &kp,
                 float2 t = make_float2(r == c ? 1.f : 0.f, 0.f);
 
                 if (r < N) {
-                    if (est && c < N)
+                    /* T in LDS (k_link): the column loads above; the queue
+                       kernels read T from global memory, one exact read
+                       per entry */
+                    if constexpr (INV == INV_CHOLESKY) {
+                        if (est && c < N)
+                            t = TCol<N, G>(tp, c).get(m, r);
+                    }
+                    else if (est && c < N)
                         t = load_t<N>(tp, r, c);
                     t = make_float2(t.x * inv[m][r], t.y * inv[m][r]);
                 }
@@ -199,47 +250,89 @@ __device__ __forceinline__ float link_group(const KParams &kp,
     wave_sync();
 
     /* gamma = |sum_{i<j} unit(T_ij) conj(z_i) z_j| / (N(N-1)/2), unit(0)
-       = 1 (tcoh.cl:726-744). Column j has j terms; lane lg takes the
-       columns c1 = p and c2 = N - 1 - p of pairs p = lg, lg + G, ..., so
-       every lane sums N - 1 terms per pair (the middle column of an odd N
-       once). */
+       = 1 (tcoh.cl:726-744). Row r of the packed triangle holds its N-1-r
+       entries T_{r,r+1..N-1} contiguously; lane lg takes the rows r1 = p
+       and r2 = N - 2 - p of pairs p = lg, lg + G, ..., N entries per lane
+       in all. Term t is entry t of row r1 for t < N-1-r1, else entry
+       t-(N-1-r1) of row r2, whose column is t itself: each is a lane base
+       plus a compile-time offset. The middle row of an even N pairs with
+       itself and counts once (its second part weighs zero). This where T
+       is in LDS (EVD, and EMI in k_link); the queue kernels, which read T
+       from global memory, take the columns below. */
     float gr = 0.0f, gim = 0.0f;
+    constexpr bool T_IN_LDS = !EMI || INV == INV_CHOLESKY;
+    constexpr int NPR = N / 2; /* row pairs of rows 0 .. N-2 */
+
+#pragma unroll
+    for (int pp = 0; pp < (NPR + G - 1) / G; pp++) {
+        const int r1 = lg + pp * G, r2 = N - 2 - r1, n1 = N - 1 - r1;
+
+        if (!T_IN_LDS || !est || pp * G >= NPR || r1 >= NPR)
+            continue;
+        const float2 za = sh.z[r1];
+        const float2 zb = r1 != r2 ? sh.z[r2] : make_float2(0.f, 0.f);
+        const float2 *t1 = tp + upper_index(N, r1, r1 + 1);
+        const float2 *t2 = tp + upper_index(N, r2, r2 + 1) - n1;
+        const float2 *z1 = sh.z + r1 + 1;
+
+#pragma unroll
+        for (int t = 0; t < N; t++) {
+            const bool first = t < n1;
+            const float2 tv = (first ? t1 : t2)[t];
+            const float2 zj = (first ? z1 : sh.z)[t];
+            const float2 zi = first ? za : zb;
+            const float m2 = fmaf(tv.x, tv.x, tv.y * tv.y);
+            const float rm = __builtin_amdgcn_rsqf(m2);
+            const float ur = m2 > 0.0f ? tv.x * rm : 1.0f;
+            const float ui = m2 > 0.0f ? tv.y * rm : 0.0f;
+            const float2 w = cmulc(zi, zj);
+
+            gr = fmaf(ur, w.x, fmaf(-ui, w.y, gr));
+            gim = fmaf(ur, w.y, fmaf(ui, w.x, gim));
+        }
+    }
+    /* gamma by columns (the queue kernels): column j has j terms; lane lg
+       takes the columns c1 = p and c2 = N - 1 - p of pairs p = lg, lg + G,
+       ..., stepping the addresses, N - 1 terms a lane (the middle column of
+       an odd N once). */
     constexpr int NPAIR = (N + 1) / 2;
 
 #pragma unroll
     for (int pp = 0; pp < (NPAIR + G - 1) / G; pp++) {
         const int c1 = lg + pp * G, c2 = N - 1 - c1;
 
-        if (!est || pp * G >= NPAIR || c1 >= NPAIR)
-            continue;
-        const float2 z1 = sh.z[c1], z2 = sh.z[c2];
-        /* T_{i,col} at upper_index(N, i, col), stepped incrementally: col - 1
-           at i = 0, then + (N - 2 - i) per row */
-        int addr = c1 - 1, stp = N - 2, i = 0;
+        if (!T_IN_LDS) {
+            if (!est || pp * G >= NPAIR || c1 >= NPAIR)
+                continue;
+            const float2 z1 = sh.z[c1], z2 = sh.z[c2];
+            /* T_{i,col} at upper_index(N, i, col): col - 1 at i = 0, then
+               + (N - 2 - i) per row */
+            int addr = c1 - 1, stp = N - 2, i = 0;
 
 #pragma unroll 4
-        for (int t = 0; t < N - 1; t++) {
-            const bool first = t < c1;
+            for (int t = 0; t < N - 1; t++) {
+                const bool first = t < c1;
 
-            if (t == c1) {
-                addr = c2 - 1;
-                stp = N - 2;
-                i = 0;
+                if (t == c1) {
+                    addr = c2 - 1;
+                    stp = N - 2;
+                    i = 0;
+                }
+                const float2 zc = first ? z1 : z2;
+                const float2 tv = tp[addr];
+                const float m2 = fmaf(tv.x, tv.x, tv.y * tv.y);
+                const float rm = __builtin_amdgcn_rsqf(m2);
+                const float ur = m2 > 0.0f ? tv.x * rm : 1.0f;
+                const float ui = m2 > 0.0f ? tv.y * rm : 0.0f;
+                const float2 w = cmulc(sh.z[i], zc);
+                const float wt = (first || c1 != c2) ? 1.0f : 0.0f;
+
+                gr = fmaf(wt, fmaf(ur, w.x, -ui * w.y), gr);
+                gim = fmaf(wt, fmaf(ur, w.y, ui * w.x), gim);
+                addr += stp;
+                stp -= 1;
+                i += 1;
             }
-            const float2 zc = first ? z1 : z2;
-            const float2 tv = tp[addr];
-            const float m2 = fmaf(tv.x, tv.x, tv.y * tv.y);
-            const float rm = __builtin_amdgcn_rsqf(m2);
-            const float ur = m2 > 0.0f ? tv.x * rm : 1.0f;
-            const float ui = m2 > 0.0f ? tv.y * rm : 0.0f;
-            const float2 w = cmulc(sh.z[i], zc);
-            const float wt = (first || c1 != c2) ? 1.0f : 0.0f;
-
-            gr = fmaf(wt, fmaf(ur, w.x, -ui * w.y), gr);
-            gim = fmaf(wt, fmaf(ur, w.y, ui * w.x), gim);
-            addr += stp;
-            stp -= 1;
-            i += 1;
         }
     }
     gr = gsum<G>(gr);
