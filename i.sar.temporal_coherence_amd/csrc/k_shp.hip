@@ -44,7 +44,9 @@ __device__ __forceinline__ float logf_rounded(float x)
    of valid pixels when the t-test is used. A block stages its PREP_PX
    pixels' amplitudes (contiguous in amp) in LDS with coalesced reads, then
    writes the date planes; a row of the tile is padded to n + 1 words so
-   that the per-pixel reads do not collide in LDS banks. */
+   that the per-pixel reads do not collide in LDS banks. The tile is sized
+   from n at launch (a tile for 64 dates took 33 KB a block and held a CU
+   to 4 blocks, too few loads in flight for this copy). */
 #define PREP_PX 128
 
 __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(PREP_PX) void k_prep(KParams kp,
                                                  const unsigned char *__restrict__ valid,
                                                  float2 *__restrict__ lstat)
 {
-    __shared__ float tile[PREP_PX * (64 + 1)]; /* up to 64 dates */
+    extern __shared__ float tile[]; /* PREP_PX x (n + 1) */
     const int n = kp.n, n1 = n + 1;
     const int p0 = blockIdx.x * PREP_PX;
     const int np = min(PREP_PX, kp.npad - p0);
@@ -366,7 +368,8 @@ hipError_t launch_prep(const KParams &kp, const float *amp, float *ampT,
                        const unsigned char *valid, float2 *lstat, hipStream_t s)
 {
     hipLaunchKernelGGL(k_prep, dim3((kp.npad + PREP_PX - 1) / PREP_PX), dim3(PREP_PX),
-                       0, s, kp, amp, ampT, valid, lstat);
+                       (size_t)PREP_PX * (kp.n + 1) * sizeof(float), s, kp, amp, ampT, valid,
+                       lstat);
     return hipGetLastError();
 }
 
