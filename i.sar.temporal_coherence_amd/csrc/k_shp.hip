@@ -132,18 +132,14 @@ __device__ __forceinline__ float load_date(const float *__restrict__ ampT, int p
     return ampT[((size_t)(d >> 2) * npv + pix) * 4 + (d & 3)];
 }
 
-/* ampT holds NB dates a pixel (n..NB-1 zero), so every load below is in
-   bounds. */
+/* The KS decision from a candidate's NB dates b (loaded by load_dates:
+   ampT holds NB dates a pixel, n..NB-1 zero, so every load is in bounds). */
 template <int NB>
-__device__ __forceinline__ bool ks_ok(const float (&ahi)[NB],
-                                      const float (&alo)[NB], int n, int kmax,
-                                      const float *__restrict__ ampT,
-                                      int cand, int npv)
+__device__ __forceinline__ bool ks_test(const float (&ahi)[NB], const float (&alo)[NB],
+                                        int kmax, const float (&b)[NB])
 {
-    float b[NB];
     bool ok = kmax >= 1;
 
-    load_dates<NB>(b, ampT, cand, npv);
     /* Dates without a condition carry NaN thresholds: !(b >= NaN) and
        !(NaN > b) hold, and for numbers they are b < ahi and alo <= b
        (amplitudes of valid pixels are never NaN). The tests of a plane of
@@ -228,18 +224,57 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
         var_a = st.y;
     }
 
-    for (int dy = 0; dy < kp.wa; dy++) {
+    /* candidate bit k (dy-major) of the mask and the count */
+    auto emit = [&](bool ok) {
+        word |= (uint32_t)ok << bit;
+        looks += ok;
+        if (++bit == 32) {
+            mk[wi++] = word;
+            word = 0;
+            bit = 0;
+        }
+    };
+
+    if (TEST == 0) {
+        /* Two candidates an iteration, both loaded before either is
+           tested, so that the second's loads overlap the first's compares
+           (every load is in bounds: both are tested unconditionally). */
+        const int kw = kp.wa * kp.wr;
+        int dy = 0, dx = 0;
+
+        for (int kk = 0; kk < kw; kk += 2) {
+            const int c0 = (y + dy) * kp.w + x + dx;
+
+            if (++dx == kp.wr) {
+                dx = 0;
+                dy++;
+            }
+            const bool two = kk + 1 < kw;
+            const int c1 = two ? (y + dy) * kp.w + x + dx : c0;
+
+            if (two && ++dx == kp.wr) {
+                dx = 0;
+                dy++;
+            }
+            float b0[NB], b1[NB];
+
+            load_dates<NB>(b0, ampT, c0, npv);
+            load_dates<NB>(b1, ampT, c1, npv);
+            const bool v0 = valid[c0] != 0, v1 = valid[c1] != 0;
+
+            emit(ks_test<NB>(ahi, alo, kp.ks_max, b0) & v0);
+            if (two)
+                emit(ks_test<NB>(ahi, alo, kp.ks_max, b1) & v1);
+        }
+    }
+    for (int dy = 0; TEST != 0 && dy < kp.wa; dy++) {
         const int row = (y + dy) * kp.w + x;
 
         for (int dx = 0; dx < kp.wr; dx++) {
             const int cand = row + dx;
             bool ok = valid[cand] != 0;
 
-            if (TEST == 0) {
-                /* every load is in bounds: test unconditionally */
-                ok = ks_ok<NB>(ahi, alo, n, kp.ks_max, ampT, cand, npv) & ok;
-            }
-            else if (ok) {
+            if (ok) {
                 if (TEST == 1) {
                     int ia = 0, ib = 0, m = 0;
                     float s = 0.0f;
@@ -303,13 +338,7 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                     ok = !(se > 0.0f) || fabsf(mean_a - st.x) / se <= kp.tlog_crit;
                 }
             }
-            word |= (uint32_t)ok << bit;
-            looks += ok;
-            if (++bit == 32) {
-                mk[wi++] = word;
-                word = 0;
-                bit = 0;
-            }
+            emit(ok);
         }
     }
     if (bit)
