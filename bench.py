@@ -47,7 +47,7 @@ METRIC_GEO = ("Mpixel/s at N=60, 21x7 KS+EVD, geometric phase removed: mixed-pre
 # profiles/k_link_n30_2048x2048_kernel_stats.md (FETCH_SIZE KiB x 2.0 +
 # WRITE_SIZE KiB x 1.0) x 1024 B.
 TRAFFIC_FORM = "bf16"
-TRAFFIC_N30 = (252950 * 2.0 + 601086 * 1.0) * 1024
+TRAFFIC_N30 = (257472 * 2.0 + 600416 * 1.0) * 1024
 PEAK_FP64 = 78.6e12    # MI355X FP64 vector (TFLOP/s)
 PEAK_FP32 = 157.3e12   # MI355X FP32 (TFLOP/s)
 PEAK_F16_MFMA = 2.5e15  # MI355X dense FP16/BF16 matrix
