@@ -86,7 +86,15 @@ __device__ __forceinline__ float link_group(const KParams &kp,
             for (int r = 0; r < 4 * HMat<N, CPL>::NG; r++) {
                 const float2 id = make_float2(r == lg + m * G ? 1.f : 0.f, 0.f);
 
-                A.set(m, r, use && r < N ? col.get(m, r) : id);
+                if (r < N) {
+                    /* loaded in every lane, then selected: a load under
+                       the lane condition became a branch around it */
+                    const float2 t = col.get(m, r);
+
+                    A.set(m, r, use ? t : id);
+                }
+                else
+                    A.set(m, r, id);
             }
         }
     }
