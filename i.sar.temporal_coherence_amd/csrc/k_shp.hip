@@ -235,10 +235,14 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
         }
     };
 
-    if (TEST == 0) {
-        /* Two candidates an iteration, both loaded before either is
-           tested, so that the second's loads overlap the first's compares
-           (every load is in bounds: both are tested unconditionally). */
+    /* KS up to 32 date slots: two candidates an iteration, both loaded
+       before either is tested, so that the second's loads overlap the
+       first's compares (every load is in bounds: both are tested
+       unconditionally). At 64 slots the second candidate's registers
+       would spill (N=60: 47 -> 73 ms a frame); one at a time there. */
+    constexpr bool KS_PAIRS = TEST == 0 && NB <= 32;
+
+    if (KS_PAIRS) {
         const int kw = kp.wa * kp.wr;
         int dy = 0, dx = 0;
 
@@ -267,14 +271,20 @@ __global__ __launch_bounds__(64) void k_shp(KParams kp,
                 emit(ks_test<NB>(ahi, alo, kp.ks_max, b1) & v1);
         }
     }
-    for (int dy = 0; TEST != 0 && dy < kp.wa; dy++) {
+    for (int dy = 0; !KS_PAIRS && dy < kp.wa; dy++) {
         const int row = (y + dy) * kp.w + x;
 
         for (int dx = 0; dx < kp.wr; dx++) {
             const int cand = row + dx;
             bool ok = valid[cand] != 0;
 
-            if (ok) {
+            if (TEST == 0) {
+                float b[NB];
+
+                load_dates<NB>(b, ampT, cand, npv);
+                ok = ks_test<NB>(ahi, alo, kp.ks_max, b) & ok;
+            }
+            else if (ok) {
                 if (TEST == 1) {
                     int ia = 0, ib = 0, m = 0;
                     float s = 0.0f;
