@@ -131,31 +131,67 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
             cr[r][c] = ci[r][c] = (ACC)0;
 
     if (est && own) {
+        /* Accepted candidates in window raster order (the reference's
+           summation order), one per trip: the next candidate's samples are
+           loaded before the current one is accumulated, so that the load
+           latency overlaps the multiply-adds. */
         const uint32_t *mk = mask + (size_t)p * kp.words;
-        int bit = 0;
-
-        for (int dy = 0; dy < kp.wa; dy++) {
-            for (int dx = 0; dx < kp.wr; dx++, bit++) {
-                if (!((mk[bit >> 5] >> (bit & 31)) & 1u))
-                    continue;
-                const float2 *s = slc + ((size_t)(y + dy) * kp.w + x + dx) * N;
-                float2 sr[B], sc[B];
-
-#pragma unroll
-                for (int r = 0; r < B; r++) {
-                    sr[r] = r0 + r < N ? s[r0 + r] : make_float2(0.f, 0.f);
-                    sc[r] = c0 + r < N ? s[c0 + r] : make_float2(0.f, 0.f);
-                }
-#pragma unroll
-                for (int r = 0; r < B; r++)
-#pragma unroll
-                    for (int c = 0; c < B; c++) {
-                        const ACC ar = sr[r].x, ai = sr[r].y, br = sc[c].x, bi_ = sc[c].y;
-
-                        cr[r][c] += ar * br + ai * bi_;
-                        ci[r][c] += ai * br - ar * bi_;
-                    }
+        const float inv_wr = 1.0f / (float)kp.wr;
+        uint32_t bits = mk[0];
+        int wi = 0;
+        /* bit index of the next accepted candidate, -1 past the last */
+        auto next = [&]() -> int {
+            while (bits == 0u) {
+                if (++wi >= kp.words)
+                    return -1;
+                bits = mk[wi];
             }
+            const int t = wi * 32 + __builtin_ctz(bits);
+
+            bits &= bits - 1u;
+            return t;
+        };
+        /* samples of candidate t (loads issued in every lane, from the
+           window's first candidate past the last; dates past N are read
+           at N - 1 and zeroed where they are used) */
+        auto fetch = [&](int t, float2 *fr, float2 *fc) {
+            const int tt = t < 0 ? 0 : t;
+            /* exact: (tt + 0.5) / wr is at least 0.5 / wr from an integer
+               and tt, wr <= TCOH_MAX_WINDOW */
+            const int dy = (int)(((float)tt + 0.5f) * inv_wr);
+            const float2 *s = slc + ((size_t)(y + dy) * kp.w + x + (tt - dy * kp.wr)) * N;
+
+#pragma unroll
+            for (int r = 0; r < B; r++) {
+                fr[r] = s[r0 + r < N ? r0 + r : N - 1];
+                fc[r] = s[c0 + r < N ? c0 + r : N - 1];
+            }
+        };
+        int t = next();
+        float2 nr[B], nc[B];
+
+        fetch(t, nr, nc);
+        while (t >= 0) {
+            float2 sr[B], sc[B];
+
+#pragma unroll
+            for (int r = 0; r < B; r++) {
+                sr[r] = r0 + r < N ? nr[r] : make_float2(0.f, 0.f);
+                sc[r] = c0 + r < N ? nc[r] : make_float2(0.f, 0.f);
+            }
+            t = next();
+            fetch(t, nr, nc);
+            /* keep the loads ahead of the multiply-adds */
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    const ACC ar = sr[r].x, ai = sr[r].y, br = sc[c].x, bi_ = sc[c].y;
+
+                    cr[r][c] += ar * br + ai * bi_;
+                    ci[r][c] += ai * br - ar * bi_;
+                }
         }
     }
     if (own && bi == bj) {
