@@ -58,24 +58,22 @@ __device__ __forceinline__ float2 coherence_entry(float cre, float cim,
     return make_float2(re, im);
 }
 
-/* The same in FP64 (EMI), rounded to float at the end. */
-__device__ __forceinline__ float2 coherence_entry(double cre, double cim,
-                                                  double den, int bias,
-                                                  int looks)
+/* The same in FP64 (EMI), rounded to float at the end, from
+   s = 1 / sqrt(C_ii C_jj) (0 where a diagonal sum is 0), with the bias
+   factor as one quotient: m2c / m2 = (L m2 - 1) /
+   ((L - 1) m2), and m2c > 0 exactly where that quotient is. */
+__device__ __forceinline__ float2 coherence_entry_scaled(double cre, double cim, double s,
+                                                         int bias, int looks)
 {
-    double re = 0.0, im = 0.0;
+    double re = cre * s, im = cim * s;
 
-    if (den > 0.0) {
-        re = cre / den;
-        im = cim / den;
-        if (bias && looks > 1) {
-            const double m2 = re * re + im * im;
-            const double m2c = (looks * m2 - 1.0) / (looks - 1.0);
-            const double f = (m2 > 0.0 && m2c > 0.0) ? sqrt(m2c / m2) : 0.0;
+    if (bias && looks > 1) {
+        const double m2 = re * re + im * im;
+        const double q = m2 > 0.0 ? (looks * m2 - 1.0) / ((looks - 1) * m2) : 0.0;
+        const double f = q > 0.0 ? sqrt(q) : 0.0;
 
-            re *= f;
-            im *= f;
-        }
+        re *= f;
+        im *= f;
     }
     return make_float2((float)re, (float)im);
 }
@@ -94,14 +92,16 @@ template <int G> struct PlainShape {
 
 /* Plain form, one pixel per group, sums in ACC: float (the reference's
    arithmetic) or double (EMI, whose |T|^-1 o T amplifies the rounding of
-   FP32 sums). diag: LDS scratch of N ACC values. */
+   FP32 sums). diag: LDS scratch of N ACC values (the diagonal sums; in
+   FP64 their reciprocal square roots); mw: of the pixel's kp.words mask
+   words. Called by every lane of the wavefront. */
 template <int N, int G, typename ACC>
 __device__ __forceinline__ void cov_plain_group(const KParams &kp,
                                                 const float2 *__restrict__ slc,
                                                 const uint32_t *__restrict__ mask,
                                                 int p, int cnt, bool est,
                                                 bool live, int lg, ACC *diag,
-                                                float2 *tl,
+                                                uint32_t *mw, float2 *tl,
                                                 float *__restrict__ pairs)
 {
     constexpr int NBLK = PlainShape<G>::NBLK;
@@ -122,7 +122,10 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
         bj = bi + t;
     }
     const bool own = lg < NBLK * (NBLK + 1) / 2;
-    const int r0 = bi * B, c0 = bj * B;
+    /* block bases clamped so that every date a lane reads is below N (the
+       last blocks of a row or column then overlap the ones before; both
+       lanes sum the shared entries in the same order) */
+    const int r0 = min(bi * B, N - B), c0 = min(bj * B, N - B);
 
 #pragma unroll
     for (int r = 0; r < B; r++)
@@ -130,21 +133,28 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
         for (int c = 0; c < B; c++)
             cr[r][c] = ci[r][c] = (ACC)0;
 
+    /* The pixel's mask words go to LDS first: read from global memory
+       inside the walk, each word's load made the trip wait for every load
+       in flight, the samples' included. */
+    if (live)
+        for (int w = lg; w < kp.words; w += G)
+            mw[w] = mask[(size_t)p * kp.words + w];
+    wave_sync();
     if (est && own) {
         /* Accepted candidates in window raster order (the reference's
-           summation order), one per trip: the next candidate's samples are
-           loaded before the current one is accumulated, so that the load
-           latency overlaps the multiply-adds. */
-        const uint32_t *mk = mask + (size_t)p * kp.words;
+           summation order). Three sample buffers rotate: the loads of each
+           candidate are issued two candidates before it is summed, so that
+           their latency overlaps the multiply-adds (the EMI kernel runs one
+           wavefront per SIMD, and nothing else would hide it). */
         const float inv_wr = 1.0f / (float)kp.wr;
-        uint32_t bits = mk[0];
+        uint32_t bits = mw[0];
         int wi = 0;
         /* bit index of the next accepted candidate, -1 past the last */
         auto next = [&]() -> int {
             while (bits == 0u) {
                 if (++wi >= kp.words)
                     return -1;
-                bits = mk[wi];
+                bits = mw[wi];
             }
             const int t = wi * 32 + __builtin_ctz(bits);
 
@@ -152,8 +162,7 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
             return t;
         };
         /* samples of candidate t (loads issued in every lane, from the
-           window's first candidate past the last; dates past N are read
-           at N - 1 and zeroed where they are used) */
+           window's first candidate past the last) */
         auto fetch = [&](int t, float2 *fr, float2 *fc) {
             const int tt = t < 0 ? 0 : t;
             /* exact: (tt + 0.5) / wr is at least 0.5 / wr from an integer
@@ -163,42 +172,62 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
 
 #pragma unroll
             for (int r = 0; r < B; r++) {
-                fr[r] = s[r0 + r < N ? r0 + r : N - 1];
-                fc[r] = s[c0 + r < N ? c0 + r : N - 1];
+                fr[r] = s[r0 + r];
+                fc[r] = s[c0 + r];
             }
-        };
-        int t = next();
-        float2 nr[B], nc[B];
-
-        fetch(t, nr, nc);
-        while (t >= 0) {
-            float2 sr[B], sc[B];
-
-#pragma unroll
-            for (int r = 0; r < B; r++) {
-                sr[r] = r0 + r < N ? nr[r] : make_float2(0.f, 0.f);
-                sc[r] = c0 + r < N ? nc[r] : make_float2(0.f, 0.f);
-            }
-            t = next();
-            fetch(t, nr, nc);
-            /* keep the loads ahead of the multiply-adds */
+            /* keep the loads ahead of the multiply-adds that follow */
             __builtin_amdgcn_sched_barrier(0);
+        };
+        auto sum = [&](const float2 *sr, const float2 *sc) {
 #pragma unroll
             for (int r = 0; r < B; r++)
 #pragma unroll
                 for (int c = 0; c < B; c++) {
                     const ACC ar = sr[r].x, ai = sr[r].y, br = sc[c].x, bi_ = sc[c].y;
 
-                    cr[r][c] += ar * br + ai * bi_;
-                    ci[r][c] += ai * br - ar * bi_;
+                    if constexpr (sizeof(ACC) == 8) {
+                        /* each FP32 product is exact in FP64: one
+                           rounding a term */
+                        cr[r][c] = fma(ai, bi_, fma(ar, br, cr[r][c]));
+                        ci[r][c] = fma(-ar, bi_, fma(ai, br, ci[r][c]));
+                    }
+                    else {
+                        cr[r][c] += ar * br + ai * bi_;
+                        ci[r][c] += ai * br - ar * bi_;
+                    }
                 }
+        };
+        float2 ar_[B], ac_[B], br_[B], bc_[B], cr_[B], cc_[B];
+        int ta = next(), tb, tc;
+
+        fetch(ta, ar_, ac_);
+        tb = next();
+        fetch(tb, br_, bc_);
+        while (ta >= 0) {
+            tc = next();
+            fetch(tc, cr_, cc_);
+            sum(ar_, ac_);
+            if (tb < 0)
+                break;
+            ta = next();
+            fetch(ta, ar_, ac_);
+            sum(br_, bc_);
+            if (tc < 0)
+                break;
+            tb = next();
+            fetch(tb, br_, bc_);
+            sum(cr_, cc_);
         }
     }
     if (own && bi == bj) {
 #pragma unroll
         for (int r = 0; r < B; r++)
             if (r0 + r < N)
-                diag[r0 + r] = cr[r][r];
+                /* FP64 (EMI): the reciprocal square root, so that each
+                   entry takes two products instead of two divisions */
+                diag[r0 + r] = sizeof(ACC) == 8
+                                   ? (cr[r][r] > (ACC)0 ? (ACC)1 / sqrt(cr[r][r]) : (ACC)0)
+                                   : cr[r][r];
     }
     wave_sync();
     if (!live || !own)
@@ -220,13 +249,13 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
                     pairs[(size_t)pidx * plane + p] = __builtin_nanf("");
                 continue;
             }
-            ACC den;
+            float2 t;
 
             if constexpr (sizeof(ACC) == 8)
-                den = sqrt(diag[i]) * sqrt(diag[j]);
+                t = coherence_entry_scaled(cr[r][c], ci[r][c], diag[i] * diag[j], kp.bias, cnt);
             else
-                den = sqrt_fast(diag[i]) * sqrt_fast(diag[j]);
-            const float2 t = coherence_entry(cr[r][c], ci[r][c], den, kp.bias, cnt);
+                t = coherence_entry(cr[r][c], ci[r][c], sqrt_fast(diag[i]) * sqrt_fast(diag[j]),
+                                    kp.bias, cnt);
 
             tl[upper_index(N, i, j)] = t;
             if (pidx >= 0)

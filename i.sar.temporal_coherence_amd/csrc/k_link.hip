@@ -33,10 +33,11 @@ template <int N> struct LinkShape {
 template <int N> static int link_scratch_bytes(const KParams &kp, int precision)
 {
     typedef LinkShape<N> S;
-    /* EMI always takes the plain form with FP64 sums */
-    const int cov = kp.emi ? S::PPW * N * 8
+    /* EMI always takes the plain form with FP64 sums; the plain form
+       keeps its diagonal sums and the pixels' mask words in LDS */
+    const int cov = kp.emi ? S::PPW * (N * 8 + kp.words * 4)
                     : precision ? mfma_scratch_bytes(S::PPW, kp.wa * kp.wr, kp.words, S::NB)
-                                : S::PPW * N * 4;
+                                : S::PPW * (N * 4 + kp.words * 4);
 
     return ((cov > S::EIG_BYTES ? cov : S::EIG_BYTES) + 15) & ~15;
 }
@@ -116,13 +117,15 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
     const bool est = live && cnt >= kp.min_looks;
 
     if constexpr (EMI)
-        cov_plain_group<N, G, double>(kp, slc, mask, p, cnt, est, live, lg,
-                                      reinterpret_cast<double *>(scratch) + gi * N,
-                                      tl + gi * NT, pairs);
+        cov_plain_group<N, G, double>(
+            kp, slc, mask, p, cnt, est, live, lg, reinterpret_cast<double *>(scratch) + gi * N,
+            reinterpret_cast<uint32_t *>(scratch + PPW * N * 8) + gi * kp.words, tl + gi * NT,
+            pairs);
     else if (cov == 0)
-        cov_plain_group<N, G, float>(kp, slc, mask, p, cnt, est, live, lg,
-                                     reinterpret_cast<float *>(scratch) + gi * N,
-                                     tl + gi * NT, pairs);
+        cov_plain_group<N, G, float>(
+            kp, slc, mask, p, cnt, est, live, lg, reinterpret_cast<float *>(scratch) + gi * N,
+            reinterpret_cast<uint32_t *>(scratch + PPW * N * 4) + gi * kp.words, tl + gi * NT,
+            pairs);
     else if (cov == 1)
         cov_mfma_band<N, NB, PPW, 0>(kp, hl, ex, mask, count, row0 + r, tx * PPW, row0, nrows,
                                      scratch, tl, pairs);
