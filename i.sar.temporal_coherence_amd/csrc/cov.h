@@ -144,8 +144,8 @@ __device__ __forceinline__ void cov_plain_group(const KParams &kp,
         /* Accepted candidates in window raster order (the reference's
            summation order). Three sample buffers rotate: the loads of each
            candidate are issued two candidates before it is summed, so that
-           their latency overlaps the multiply-adds (the EMI kernel runs one
-           wavefront per SIMD, and nothing else would hide it). */
+           their latency overlaps the multiply-adds (the FP64 form leaves
+           room for two wavefronts per SIMD at most, which hide little). */
         const float inv_wr = 1.0f / (float)kp.wr;
         uint32_t bits = mw[0];
         int wi = 0;

@@ -86,7 +86,7 @@ __device__ __forceinline__ void store_phases(const KParams &kp, const EigLds<N> 
    wavefront hides more than the spills cost (N=45: 49.7 -> 32.8 ms, N=60:
    33.9 -> 29.8 ms on 768^2 and 512^2 frames); EMI keeps one (512). */
 template <int N, int EMI>
-__global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
+__global__ __launch_bounds__(64, EMI && N > 32 ? 1 : 2) void k_link(KParams kp,
                                                const float2 *__restrict__ slc,
                                                const uint2 *__restrict__ hl,
                                                const short *__restrict__ ex,
@@ -100,9 +100,11 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
 {
     typedef LinkShape<N> S;
     constexpr int G = S::G, PPW = S::PPW, NT = S::NT, NB = S::NB;
+    /* EMI up to 32 dates: k_cov_emi has left T in the band's store */
+    constexpr bool SPLIT = EMI && N <= 32;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
     float2 *tl = reinterpret_cast<float2 *>(lds);
-    unsigned char *scratch = lds + S::TL_BYTES;
+    unsigned char *scratch = lds + (SPLIT ? 0 : S::TL_BYTES);
     const int gi = threadIdx.x / G, lg = threadIdx.x % G;
     EmiLds<N> *em = reinterpret_cast<EmiLds<N> *>(scratch + scratch_bytes +
                                                   (EMI ? gi * S::CH_STRIDE : 0));
@@ -116,7 +118,9 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
     const int cnt = live ? count[p] : -1;
     const bool est = live && cnt >= kp.min_looks;
 
-    if constexpr (EMI)
+    if constexpr (SPLIT) {
+    }
+    else if constexpr (EMI)
         cov_plain_group<N, G, double>(
             kp, slc, mask, p, cnt, est, live, lg, reinterpret_cast<double *>(scratch) + gi * N,
             reinterpret_cast<uint32_t *>(scratch + PPW * N * 8) + gi * kp.words, tl + gi * NT,
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
     const int bp = r * kp.cols + (live ? x : 0); /* band pixel */
     bool deferred;
 
-    if constexpr (EMI) {
+    if constexpr (EMI && !SPLIT) {
         /* T of the estimated pixel for the queues and the refinement */
         if (est) {
             float2 *dst = q.t + (size_t)bp * NT;
@@ -149,8 +153,8 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
     /* (only live groups store an eigenvector: a group past the row's end
        has bp of the row's first pixel) */
     bool refine = false;
-    const float gam = link_group<N, G, EMI, INV_CHOLESKY>(
-        kp, tl + gi * NT, est, lg, sh[gi], *em, deferred,
+    const float gam = link_group<N, G, EMI, SPLIT ? INV_CHOLESKY_GLOBAL : INV_CHOLESKY>(
+        kp, SPLIT ? q.t + (size_t)bp * NT : tl + gi * NT, est, lg, sh[gi], *em, deferred,
         EMI && live ? q.z + (size_t)bp * N : nullptr, EMI ? &refine : nullptr);
 
     if constexpr (EMI) {
@@ -167,6 +171,47 @@ __global__ __launch_bounds__(64, EMI ? 1 : 2) void k_link(KParams kp,
         coh[p] = gam;
     if (live && !deferred)
         store_phases<N, G>(kp, sh[gi], p, lg, !__builtin_isnan(gam));
+}
+
+/* EMI up to 32 dates, first of two kernels: the band's FP64 covariance
+   (cov_plain_group<double>), T of each estimated pixel to the band's store
+   q.t, and the pair maps. Apart, the FP64 sums and the linking each fit
+   256 registers, and both kernels run two wavefronts per SIMD (together,
+   one); k_link then reads T from q.t. */
+template <int N>
+__global__ __launch_bounds__(64, 2) void k_cov_emi(KParams kp, const float2 *__restrict__ slc,
+                                                   const uint32_t *__restrict__ mask,
+                                                   const int *__restrict__ count,
+                                                   float *__restrict__ pairs, int row0,
+                                                   int nrows, EmiQueue q)
+{
+    typedef LinkShape<N> S;
+    constexpr int G = S::G, PPW = S::PPW, NT = S::NT;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    float2 *tl = reinterpret_cast<float2 *>(lds);
+    unsigned char *scratch = lds + S::TL_BYTES;
+    const int gi = threadIdx.x / G, lg = threadIdx.x % G;
+    int r, tx;
+
+    if (!xcd_tile(blockIdx.x, nrows, (kp.cols + PPW - 1) / PPW, kp.link_stripe, &r, &tx))
+        return;
+    const int x = tx * PPW + gi;
+    const bool live = x < kp.cols;
+    const int p = (row0 + r) * kp.cols + (live ? x : 0);
+    const int cnt = live ? count[p] : -1;
+    const bool est = live && cnt >= kp.min_looks;
+
+    cov_plain_group<N, G, double>(
+        kp, slc, mask, p, cnt, est, live, lg, reinterpret_cast<double *>(scratch) + gi * N,
+        reinterpret_cast<uint32_t *>(scratch + PPW * N * 8) + gi * kp.words, tl + gi * NT,
+        pairs);
+    wave_sync();
+    if (est) {
+        float2 *dst = q.t + ((size_t)r * kp.cols + x) * NT;
+
+        for (int t = lg; t < NT; t += G)
+            dst[t] = tl[gi * NT + t];
+    }
 }
 
 /* Queue 1 (pixels whose |T| failed the Cholesky test), PPW to a
@@ -343,10 +388,27 @@ hipError_t launch_link_n(const KParams &kp, const float2 *slc, const uint2 *hl,
     typedef LinkShape<N> S;
     const dim3 grid(xcd_grid(nrows, (kp.cols + S::PPW - 1) / S::PPW, kp.link_stripe));
     const int scratch = link_scratch_bytes<N>(kp, precision);
-    const size_t lds = (size_t)S::TL_BYTES + scratch + (kp.emi ? S::PPW * S::CH_STRIDE : 0);
+    const bool split = kp.emi && N <= 32;
+    /* the split EMI kernel keeps no T in LDS: its eigen area and the
+       Cholesky factors only */
+    const size_t lds = (split ? 0 : (size_t)S::TL_BYTES) + scratch +
+                       (kp.emi ? S::PPW * S::CH_STRIDE : 0);
 
     if (kp.emi && !(q.t && q.z && q.ref && q.pix && q.n && q.next))
         return hipErrorInvalidValue;
+    if constexpr (N <= 32) {
+        if (split) {
+            const size_t lc = (size_t)S::TL_BYTES + ((S::PPW * (N * 8 + kp.words * 4) + 15) & ~15);
+            const hipError_t e = lds_limit(reinterpret_cast<const void *>(&k_cov_emi<N>), lc);
+
+            if (e != hipSuccess)
+                return e;
+            hipLaunchKernelGGL(k_cov_emi<N>, grid, dim3(WAVE), lc, s, kp, slc, mask, count,
+                               pairs, row0, nrows, q);
+            if (hipGetLastError() != hipSuccess)
+                return hipErrorLaunchFailure;
+        }
+    }
 #define TCOH_LINK_LAUNCH(EMI)                                                   \
     do {                                                                        \
         const hipError_t e =                                                    \

@@ -10,7 +10,7 @@
 
 /* |T|^-1 of EMI by INV: the Cholesky of k_link, the Gauss-Jordan inverse
    of the first queue kernel, the Jacobi pseudo-inverse of the second. */
-enum { INV_CHOLESKY = 0, INV_GAUSS_JORDAN = 1, INV_JACOBI = 2 };
+enum { INV_CHOLESKY = 0, INV_GAUSS_JORDAN = 1, INV_JACOBI = 2, INV_CHOLESKY_GLOBAL = 3 };
 
 /* Column c of the pixel's T (lane c of a group of G lanes) from the packed
    strict upper triangle: T_rc at upper_index(N, r, c) = K(r) + c above
@@ -149,7 +149,7 @@ __device__ __forceinline__ float link_group(const KParams &kp,
         float inv[CPL][N];
         bool ok = true;
 
-        if constexpr (INV == INV_CHOLESKY)
+        if constexpr (INV == INV_CHOLESKY || INV == INV_CHOLESKY_GLOBAL)
             ok = chol_inverse<N, G, CPL>(tp, est, em, lg, inv);
         else if constexpr (INV == INV_GAUSS_JORDAN)
             ok = gj_inverse<N, G, CPL>(tp, est, em, lg, inv);
